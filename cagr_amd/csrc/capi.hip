@@ -1,0 +1,308 @@
+// C ABI of the cagr_amd library: argument checks, operator -> kernel family
+// mapping, and the calls into the build / upward / traversal code.
+#include "common.h"
+#include "tree_export.h"
+#include <cstring>
+#include <new>
+
+namespace cagr {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+// Morton order of arbitrary query points (implemented in tree_build.hip)
+int morton_order(const void* pts, int dtype, int64_t n, int32_t* order, void* scratch,
+                 hipStream_t st);
+size_t morton_order_scratch(int64_t n);
+
+namespace {
+
+struct OpInfo {
+    int family;   // 0 vector, 1 scalar sets
+    int K;        // sets
+    int dir0;     // convection axis of set 0 (scalar family)
+    int in_dim, out_dim;
+    bool convects;
+};
+
+bool op_info(int op, OpInfo* o) {
+    switch (op) {
+        case CAGR_OP_A: case CAGR_OP_A_CPU: *o = {0, 1, 0, 3, 1, false}; return true;
+        case CAGR_OP_G: *o = {0, 1, 0, 3, 3, false}; return true;
+        case CAGR_OP_A21: case CAGR_OP_A22: case CAGR_OP_A23: case CAGR_OP_A3:
+            *o = {0, 1, 0, 3, 1, true}; return true;
+        case CAGR_OP_A2: *o = {0, 1, 0, 3, 3, true}; return true;
+        case CAGR_OP_AT: *o = {1, 1, 0, 1, 3, false}; return true;
+        case CAGR_OP_A2T1: case CAGR_OP_A2T2: case CAGR_OP_A2T3:
+            *o = {1, 1, op - CAGR_OP_A2T1, 1, 3, true}; return true;
+        case CAGR_OP_A2T_SUM: *o = {1, 3, 0, 3, 3, true}; return true;
+    }
+    return false;
+}
+
+int check_common(const cagr_tree* t, int op, int precision, int dtype, OpInfo* oi) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    CAGR_REQUIRE(op_info(op, oi), CAGR_ERR_ARG, "unknown operator code");
+    CAGR_REQUIRE(precision == CAGR_PREC_FP64 || precision == CAGR_PREC_FP32C, CAGR_ERR_ARG,
+                 "precision must be CAGR_PREC_FP64 or CAGR_PREC_FP32C");
+    CAGR_REQUIRE(dtype == CAGR_F32 || dtype == CAGR_F64, CAGR_ERR_ARG,
+                 "dtype must be CAGR_F32 or CAGR_F64");
+    int dev = -1;
+    CAGR_HIP(hipGetDevice(&dev));
+    CAGR_REQUIRE(dev == t->device, CAGR_ERR_STATE,
+                 "current HIP device differs from the device the tree was built on");
+    return CAGR_OK;
+}
+
+int run_upward_for(cagr_tree* t, int op, int precision, const OpInfo& oi, const void* attrs,
+                   int dtype, const void* weights, double an, hipStream_t st) {
+    const double an_rec = oi.convects ? an : 0.0;
+    return upward(t, oi.family, precision, oi.K, oi.dir0, attrs, dtype, weights, an_rec, st);
+}
+
+int apply_impl(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+               const void* weights, const void* widths, int continuous, double an, void* out,
+               void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, precision, dtype, &oi);
+    if (rc) return rc;
+    if (t->n == 0) return CAGR_OK;
+    CAGR_REQUIRE(attrs && widths && out, CAGR_ERR_ARG, "null attrs / widths / out");
+    hipStream_t st = (hipStream_t)stream;
+    rc = run_upward_for(t, op, precision, oi, attrs, dtype, weights, an, st);
+    if (rc) return rc;
+    Queries q;
+    q.mode = 0;
+    q.nq = t->n;
+    q.widths = widths;
+    q.wdtype = dtype;
+    return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
+}
+
+}  // namespace
+}  // namespace cagr
+
+using namespace cagr;
+
+extern "C" {
+
+const char* cagr_last_error(void) { return g_err.c_str(); }
+
+int cagr_version(void) { return 1; }
+
+int cagr_tree_build(const void* points, int64_t n, int dtype, int max_depth, int leaf_size,
+                    double theta, void* stream, cagr_tree** out) {
+    CAGR_REQUIRE(out != nullptr, CAGR_ERR_ARG, "null output handle pointer");
+    *out = nullptr;
+    CAGR_REQUIRE(dtype == CAGR_F32 || dtype == CAGR_F64, CAGR_ERR_ARG,
+                 "points dtype must be CAGR_F32 or CAGR_F64");
+    CAGR_REQUIRE(n >= 0 && n < (int64_t(1) << 31) - 1, CAGR_ERR_ARG,
+                 "number of points must be in [0, 2^31 - 1)");
+    CAGR_REQUIRE(n == 0 || points != nullptr, CAGR_ERR_ARG, "null points");
+    CAGR_REQUIRE(max_depth >= 0 && max_depth <= kLevels, CAGR_ERR_ARG,
+                 "max_depth must be in [0, 15] (ALLOWED_MAX_DEPTH)");
+    CAGR_REQUIRE(leaf_size >= 1 && leaf_size <= 64, CAGR_ERR_ARG, "leaf_size must be in [1, 64]");
+    CAGR_REQUIRE(theta > 0.0 && theta <= 2.0, CAGR_ERR_ARG, "theta must be in (0, 2]");
+    cagr_tree* t = new (std::nothrow) cagr_tree;
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_STATE, "out of host memory");
+    const int rc = build_tree(t, points, n, dtype, max_depth, leaf_size, theta, (hipStream_t)stream);
+    if (rc) {
+        const std::string keep = g_err;
+        cagr_tree_destroy(t);
+        g_err = keep;
+        return rc;
+    }
+    *out = t;
+    return CAGR_OK;
+}
+
+int cagr_tree_destroy(cagr_tree* t) {
+    if (!t) return CAGR_OK;
+    (void)hipFree(t->pts64);
+    (void)hipFree(t->pts32);
+    (void)hipFree(t->perm);
+    (void)hipFree(t->keys);
+    (void)hipFree(t->topo);
+    (void)hipFree(t->order);
+    (void)hipFree(t->src);
+    (void)hipFree(t->nodes);
+    (void)hipFree(t->w64);
+    (void)hipFree(t->r64);
+    (void)hipFree(t->m64);
+    (void)hipFree(t->qws);
+    delete t;
+    return CAGR_OK;
+}
+
+int cagr_tree_info(const cagr_tree* t, int64_t* num_nodes, int64_t* num_leaves, int64_t* depth,
+                   int64_t* list_len) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    if (num_nodes) *num_nodes = t->m;
+    if (num_leaves) *num_leaves = t->n_leaves;
+    if (depth) *depth = t->depth;
+    if (list_len) {
+        if (t->m == 0) {
+            *list_len = 0;
+        } else {
+            std::vector<TopoRec> topo(t->m);
+            CAGR_HIP(hipMemcpy(topo.data(), t->topo, t->m * sizeof(TopoRec), hipMemcpyDeviceToHost));
+            int64_t s = 0;
+            for (const auto& r : topo) s += (int64_t)r.end - r.begin;
+            *list_len = s;
+        }
+    }
+    return CAGR_OK;
+}
+
+int cagr_tree_export(const cagr_tree* t, int64_t* parent, int64_t* children, uint8_t* is_leaf,
+                     double* half_w, int64_t* num_points, int64_t* node2point_index,
+                     int64_t* node2point_indexstart) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    CAGR_REQUIRE(parent && children && is_leaf && half_w && num_points && node2point_indexstart,
+                 CAGR_ERR_ARG, "null export buffer");
+    if (t->m == 0) return CAGR_OK;
+    std::vector<TopoRec> topo(t->m);
+    std::vector<int32_t> perm(t->n);
+    std::vector<uint64_t> keys(t->n);
+    CAGR_HIP(hipDeviceSynchronize());
+    CAGR_HIP(hipMemcpy(topo.data(), t->topo, t->m * sizeof(TopoRec), hipMemcpyDeviceToHost));
+    CAGR_HIP(hipMemcpy(perm.data(), t->perm, t->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    CAGR_HIP(hipMemcpy(keys.data(), t->keys, t->n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    export_reference_layout(topo.data(), t->m, perm.data(), keys.data(), parent, children, is_leaf,
+                            half_w, num_points, node2point_index, node2point_indexstart);
+    return CAGR_OK;
+}
+
+int cagr_tree_permutation(const cagr_tree* t, int32_t* perm_host) {
+    CAGR_REQUIRE(t != nullptr && perm_host != nullptr, CAGR_ERR_ARG, "null argument");
+    if (t->n == 0) return CAGR_OK;
+    CAGR_HIP(hipDeviceSynchronize());
+    CAGR_HIP(hipMemcpy(perm_host, t->perm, t->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return CAGR_OK;
+}
+
+int cagr_upward(cagr_tree* t, const void* attrs, int dtype, int attr_dim, const void* weights,
+                double* node_attrs, double* node_reppoints, double* node_weights, void* stream) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    CAGR_REQUIRE(attr_dim == 1 || attr_dim == 3, CAGR_ERR_ARG, "attr_dim must be 1 or 3");
+    CAGR_REQUIRE(dtype == CAGR_F32 || dtype == CAGR_F64, CAGR_ERR_ARG, "bad dtype");
+    if (t->m == 0) return CAGR_OK;
+    CAGR_REQUIRE(attrs && node_attrs && node_reppoints && node_weights, CAGR_ERR_ARG,
+                 "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int rc = upward(t, attr_dim == 3 ? 0 : 1, CAGR_PREC_FP64, 1, 0, attrs, dtype, weights,
+                          0.0, st);
+    if (rc) return rc;
+    CAGR_HIP(hipMemcpyAsync(node_attrs, t->m64, t->m * attr_dim * sizeof(double),
+                            hipMemcpyDeviceToDevice, st));
+    CAGR_HIP(hipMemcpyAsync(node_reppoints, t->r64, t->m * 3 * sizeof(double),
+                            hipMemcpyDeviceToDevice, st));
+    CAGR_HIP(hipMemcpyAsync(node_weights, t->w64, t->m * sizeof(double), hipMemcpyDeviceToDevice, st));
+    return CAGR_OK;
+}
+
+int cagr_apply(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+               const void* widths, int continuous, double an, void* out, void* stream) {
+    return apply_impl(t, op, precision, attrs, dtype, nullptr, widths, continuous, an, out, stream);
+}
+
+int cagr_apply_weighted(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+                        const void* weights, const void* widths, int continuous, double an,
+                        void* out, void* stream) {
+    return apply_impl(t, op, precision, attrs, dtype, weights, widths, continuous, an, out, stream);
+}
+
+int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+                       const void* queries, int qdtype, int64_t nq, const void* widths,
+                       double width, int continuous, double an, void* out, void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, precision, dtype, &oi);
+    if (rc) return rc;
+    CAGR_REQUIRE(qdtype == CAGR_F32 || qdtype == CAGR_F64, CAGR_ERR_ARG, "bad query dtype");
+    CAGR_REQUIRE(nq >= 0 && nq < (int64_t(1) << 31) - 1, CAGR_ERR_ARG, "bad query count");
+    if (nq == 0) return CAGR_OK;
+    CAGR_REQUIRE(queries && out, CAGR_ERR_ARG, "null queries / out");
+    hipStream_t st = (hipStream_t)stream;
+    if (t->n == 0) {   // no sources: every output is zero
+        CAGR_HIP(hipMemsetAsync(out, 0, nq * oi.out_dim * (dtype == CAGR_F32 ? 4 : 8), st));
+        return CAGR_OK;
+    }
+    CAGR_REQUIRE(attrs, CAGR_ERR_ARG, "null attrs");
+    const size_t need = nq * sizeof(int32_t) + morton_order_scratch(nq);
+    if (need > t->ws_queries) {   // grow-only query workspace
+        CAGR_HIP(hipStreamSynchronize(st));
+        (void)hipFree(t->qws);
+        t->qws = nullptr;
+        t->ws_queries = 0;
+        CAGR_HIP(hipMalloc(&t->qws, need));
+        t->ws_queries = need;
+    }
+    int32_t* order = (int32_t*)t->qws;
+    rc = morton_order(queries, qdtype, nq, order, (char*)t->qws + nq * sizeof(int32_t), st);
+    if (rc) return rc;
+    rc = run_upward_for(t, op, precision, oi, attrs, dtype, nullptr, an, st);
+    if (rc) return rc;
+    Queries q;
+    q.mode = 1;
+    q.nq = nq;
+    q.q = queries;
+    q.qdtype = qdtype;
+    q.qorder = order;
+    q.widths = widths;
+    q.wdtype = dtype;
+    q.width = width;
+    return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
+}
+
+int cagr_grid_query(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+                    const double* lo, const double* step, const int64_t* dims, double width,
+                    int continuous, double an, void* out, void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, precision, dtype, &oi);
+    if (rc) return rc;
+    CAGR_REQUIRE(op == CAGR_OP_A || op == CAGR_OP_A_CPU || op == CAGR_OP_A3 ||
+                     (op >= CAGR_OP_A21 && op <= CAGR_OP_A23),
+                 CAGR_ERR_ARG, "grid query supports A, A_CPU, A21..A23 and A3");
+    CAGR_REQUIRE(lo && step && dims && out, CAGR_ERR_ARG, "null argument");
+    for (int d = 0; d < 3; ++d)
+        CAGR_REQUIRE(dims[d] >= 0 && dims[d] < (int64_t(1) << 20), CAGR_ERR_ARG, "bad grid dims");
+    const int64_t total = dims[0] * dims[1] * dims[2];
+    if (total == 0) return CAGR_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (t->n == 0) {
+        CAGR_HIP(hipMemsetAsync(out, 0, total * (dtype == CAGR_F32 ? 4 : 8), st));
+        return CAGR_OK;
+    }
+    CAGR_REQUIRE(attrs, CAGR_ERR_ARG, "null attrs");
+    rc = run_upward_for(t, op, precision, oi, attrs, dtype, nullptr, an, st);
+    if (rc) return rc;
+    Queries q;
+    q.mode = 2;
+    for (int d = 0; d < 3; ++d) {
+        q.glo[d] = lo[d];
+        q.gstep[d] = step[d];
+        q.gdim[d] = dims[d];
+    }
+    q.width = width;
+    return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
+}
+
+int cagr_count(cagr_tree* t, int op, const void* attrs, int dtype, const void* weights,
+               int64_t* n_far, int64_t* n_near, int64_t* n_visit, void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, CAGR_PREC_FP64, dtype, &oi);
+    if (rc) return rc;
+    if (t->n == 0) return CAGR_OK;
+    CAGR_REQUIRE(attrs, CAGR_ERR_ARG, "null attrs");
+    CAGR_REQUIRE(n_far || n_near || n_visit, CAGR_ERR_ARG, "no counter output given");
+    hipStream_t st = (hipStream_t)stream;
+    rc = run_upward_for(t, op, CAGR_PREC_FP64, oi, attrs, dtype, weights, 1.0, st);
+    if (rc) return rc;
+    Queries q;
+    q.mode = 0;
+    q.nq = t->n;
+    q.width = 0.0;
+    return traverse(t, op, CAGR_PREC_FP64, q, 0, 1.0, nullptr, dtype, n_far, n_near, n_visit, st);
+}
+
+}  // extern "C"

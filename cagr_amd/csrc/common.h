@@ -1,0 +1,120 @@
+// Shared definitions of the cagr_amd HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "tree_logic.h"
+#include "../../include/cagr_amd.h"
+
+namespace cagr {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+#define CAGR_HIP(expr)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::cagr::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+            return CAGR_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+#define CAGR_LAUNCHED()                                                             \
+    do {                                                                            \
+        hipError_t e_ = hipGetLastError();                                          \
+        if (e_ != hipSuccess) {                                                     \
+            ::cagr::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return CAGR_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+#define CAGR_REQUIRE(cond, code, msg)                                               \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            ::cagr::set_error(msg);                                                 \
+            return code;                                                            \
+        }                                                                           \
+    } while (0)
+
+// ---------------------------------------------------------------- records
+// Source (point) records, one per sorted point, rebuilt per call; node
+// records, one per node (per set for the scalar family), rebuilt by the
+// upward pass.  All are read with wave-uniform (scalar) loads in the
+// traversal, so they are packed point-major.
+
+// vector family (attr_dim 3: A, A_cpu, G, A2k, A2 fused, A3)
+struct alignas(16) SrcV64 { double y[3]; double mu[3]; };
+struct alignas(16) SrcV32 { float y[3]; float mu[3]; float e[3]; float q[3]; };
+//   e[k] = exp(-An*y_k/2), q[k] = An/2 * e[k] * mu[k]
+struct alignas(16) NodeV64 { double c[3]; double m[3]; };
+struct alignas(16) NodeV32 { double c[3]; float m[3]; float e[3]; float q[3]; float pad; };
+//   c = weighted centre (FP64, decides far/near), m = attr sum, e/q as above at c
+
+// scalar family (attr_dim 1 per set: AT, A2Tk, fused sum_k A2Tk)
+template <int K> struct alignas(16) SrcS64 { double y[3]; double s[K]; };
+template <int K> struct alignas(16) SrcS32 { float y[3]; float a[K]; };
+//   a[k] = s_k * exp(-An*y_{dir_k}/2)   (= s_k for AT)
+struct alignas(16) NodeS64 { double c[3]; double s; };
+struct alignas(16) NodeS32 { double c[3]; float s; float e; };
+//   e = exp(-An*c_{dir}/2) at this set's centre
+
+}  // namespace cagr
+
+// ---------------------------------------------------------------- tree handle
+struct cagr_tree {
+    int device = 0;
+    int64_t n = 0;            // points
+    int64_t m = 0;            // nodes
+    int64_t n_leaves = 0;
+    int depth = 0;            // deepest node
+    int max_depth = 15;
+    int leaf_size = 1;
+    double theta = 0.5;
+    double open_factor = 4.0; // 2 / theta
+    int point_dtype = CAGR_F32;
+
+    // device, owned
+    double* pts64 = nullptr;      // [n,3] sorted by (key, leaf-local index)
+    float* pts32 = nullptr;       // [n,3] same order
+    int32_t* perm = nullptr;      // [n] sorted position -> original index
+    uint64_t* keys = nullptr;     // [n] sorted Morton keys
+    cagr::TopoRec* topo = nullptr;// [m] preorder
+    int32_t* order = nullptr;     // [m] leaves, then internal nodes deepest level first
+    std::vector<int64_t> group_off;  // host: order[] offsets: [0]=leaves, [1+i] internal depth (14-i)
+
+    // workspace, owned (sized at build)
+    void* src = nullptr;          // n * 64 bytes
+    void* nodes = nullptr;        // m * 128 bytes
+    double* w64 = nullptr;        // [3, m]
+    double* r64 = nullptr;        // [3, m, 3]
+    double* m64 = nullptr;        // [3, m, 3]
+    size_t ws_queries = 0;        // grow-only query workspace (explicit-query sorting)
+    void* qws = nullptr;
+};
+
+namespace cagr {
+// implemented in tree_build.hip
+int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_depth,
+               int leaf_size, double theta, hipStream_t st);
+int sort_pairs_u64(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp,
+                   uint32_t* hist, int64_t n, int bits, hipStream_t st);
+int64_t sort_hist_len(int64_t n);
+// implemented in upward.hip
+int upward(cagr_tree* t, int family, int precision, int K, int dir0, const void* attrs,
+           int dtype, const void* weights, double an, hipStream_t st);
+// implemented in traverse.hip
+struct Queries {
+    int mode = 0;                 // 0 self, 1 explicit, 2 grid
+    int64_t nq = 0;
+    const void* q = nullptr;      // explicit [nq,3]
+    int qdtype = CAGR_F32;
+    const int32_t* qorder = nullptr;
+    double glo[3] = {0, 0, 0}, gstep[3] = {0, 0, 0};
+    int64_t gdim[3] = {0, 0, 0};
+    const void* widths = nullptr; // per query (self: original order), or null
+    int wdtype = CAGR_F32;
+    double width = 0.0;           // used when widths == null
+};
+int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuous, double an,
+             void* out, int odtype, int64_t* cnt_far, int64_t* cnt_near, int64_t* cnt_visit,
+             hipStream_t st);
+}  // namespace cagr

@@ -1,0 +1,715 @@
+// Treecode traversal kernels (gfx950).
+//
+// Reference semantics (CUDA-based/ext/wn_treecode/wn_treecode_cuda/wn_treecode_cuda_kernels.cu:556-1257):
+// each query walks the tree depth-first from the root; a node whose centre is
+// far (dist^2 > (open_factor*half_w)^2) contributes once through its moment,
+// a near leaf contributes every one of its points, a near internal node is
+// opened.  Kernel math: cuda_kernels.cu:79-414 (A, AT, A2_k, A2T_k, G) and
+// :694-763 (A3).
+//
+// Execution model here.  One wavefront serves 64 queries that are
+// consecutive in Morton order (the tree's own sorted points, a 4x4x4 block of
+// a grid, or Morton-sorted external queries).  The wave walks the preorder
+// node array with a single wave-uniform cursor n: every node record it reads
+// is a wave-uniform (scalar) load.  Each lane keeps `resume`, the first node
+// id it still has to consider: a lane is active at n iff resume <= n.  An
+// active lane that finds n far takes the far interaction and skips n's
+// subtree (resume = escape(n)); at a near leaf it takes the leaf's points and
+// also skips past it; at a near internal node it stays active for the
+// children.  The wave moves to n+1 if any active lane opened n, else to
+// escape(n).  Every lane therefore visits exactly the nodes of its own
+// reference DFS and takes exactly the same far/near decisions; only the order
+// of the summands differs.
+//
+// Far/near tests run in FP64 in both precisions (the decision must not
+// depend on the precision of the kernel sums); in the FP32 form a cutoff
+// test r >= w that FP32 cannot resolve (|r^2 - w^2| <= 1e-5 w^2) is redone in
+// FP64.  FP32 sums are two-level: plain partial sums per leaf and per run of
+// far interactions, folded into the running total with Neumaier's
+// compensated addition.
+//
+// The convection factor exp(An*(x-y)_k/2) factors into exp(An*x_k/2) (per
+// query) times exp(-An*y_k/2) (per source, precomputed in the source/node
+// records), so the FP32 form needs no exp per interaction.  The FP64 form
+// evaluates exp per interaction like the reference.
+#include "common.h"
+#include <climits>
+
+namespace cagr {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / 64;
+
+enum { QM_SELF = 0, QM_EXPLICIT = 1, QM_GRID = 2 };
+enum { VOP_A = 0, VOP_G = 1, VOP_A2 = 2, VOP_A3 = 3 };
+
+struct TP {
+    int64_t nq;               // lanes (grid: blocks * 64)
+    int32_t root, stop;
+    const int32_t* perm;
+    const void* q;
+    int qdtype;
+    const int32_t* qorder;
+    double glo[3], gstep[3];
+    int64_t gdim[3], gblk[3];
+    const void* widths;
+    int wdtype;
+    double width;
+    void* out;
+    int odtype;
+    double an;
+    double sign;              // A: +1 CUDA convention, -1 CPU convention
+    int kdir;                 // V: output column for A2k (-1 = all three); S: direction of set 0
+    int64_t* cnt_far;
+    int64_t* cnt_near;
+    int64_t* cnt_visit;
+};
+
+// workgroup -> logical block so that each XCD (workgroups are dealt to the 8
+// XCDs round-robin) gets one contiguous, spatially coherent range of blocks
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+__device__ inline double load_as_double(const void* p, int dtype, int64_t i) {
+    return dtype == CAGR_F32 ? (double)((const float*)p)[i] : ((const double*)p)[i];
+}
+
+__device__ inline void store_out(const TP& p, int64_t i, double v) {
+    if (p.odtype == CAGR_F32) ((float*)p.out)[i] = (float)v;
+    else ((double*)p.out)[i] = v;
+}
+
+// squared distance, each operation rounded (the reference's order:
+// d0^2 + d1^2 + d2^2 from the left)
+__device__ inline double dist2_rn(double d0, double d1, double d2) {
+#pragma clang fp contract(off)
+    return (d0 * d0 + d1 * d1) + d2 * d2;
+}
+
+template <int QM>
+__device__ inline bool load_query(const TP& p, const double* __restrict__ pts64, int64_t idx,
+                                  double x[3], double& w, int64_t& oi) {
+    bool valid;
+    if constexpr (QM == QM_SELF) {
+        valid = idx < p.nq;
+        const int64_t s = valid ? idx : 0;
+        x[0] = pts64[3 * s]; x[1] = pts64[3 * s + 1]; x[2] = pts64[3 * s + 2];
+        oi = p.perm[s];
+        w = p.widths ? load_as_double(p.widths, p.wdtype, oi) : p.width;
+    } else if constexpr (QM == QM_EXPLICIT) {
+        valid = idx < p.nq;
+        const int64_t s = valid ? idx : 0;
+        const int64_t i = p.qorder ? (int64_t)p.qorder[s] : s;
+        for (int d = 0; d < 3; ++d) x[d] = load_as_double(p.q, p.qdtype, 3 * i + d);
+        oi = i;
+        w = p.widths ? load_as_double(p.widths, p.wdtype, i) : p.width;
+    } else {
+        const int64_t wave = idx >> 6;
+        const int lane = (int)(idx & 63);
+        const int64_t bz = wave % p.gblk[2];
+        const int64_t by = (wave / p.gblk[2]) % p.gblk[1];
+        const int64_t bx = wave / (p.gblk[2] * p.gblk[1]);
+        const int64_t ix = bx * 4 + (lane >> 4), iy = by * 4 + ((lane >> 2) & 3),
+                      iz = bz * 4 + (lane & 3);
+        valid = bx < p.gblk[0] && ix < p.gdim[0] && iy < p.gdim[1] && iz < p.gdim[2];
+        x[0] = p.glo[0] + (double)ix * p.gstep[0];
+        x[1] = p.glo[1] + (double)iy * p.gstep[1];
+        x[2] = p.glo[2] + (double)iz * p.gstep[2];
+        oi = valid ? (ix * p.gdim[1] + iy) * p.gdim[2] + iz : 0;
+        w = p.width;
+    }
+    return valid;
+}
+
+// a[i] for a small runtime i without indexing a register array (which would
+// spill the array to scratch)
+template <typename T>
+__device__ inline T sel3(const T* a, int i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+
+__device__ inline void neumaier(float& s, float& c, float v) {
+    const float t = s + v;
+    c += (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
+    s = t;
+}
+
+// r >= w decided as the FP64 reference would: FP32 unless too close to call
+__device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, double xd2,
+                                   double w) {
+    const bool amb = fabsf(r2 - w2) <= 1e-5f * w2;
+    bool in = r2 < w2;
+    if (__builtin_expect(__ballot(amb) != 0, 0)) {
+        if (amb) in = !(sqrt(dist2_rn(xd0, xd1, xd2)) >= w);
+    }
+    return in;
+}
+
+// ============================================================================
+// vector family: A / A_cpu, G, A2 (all three directions), A3
+// ============================================================================
+template <int VOP, bool CONT>
+struct VAcc32 {
+    static constexpr int NO = (VOP == VOP_A) ? 1 : 3;
+    float s[NO], c[NO], lp[NO], fp[NO];
+    float P, Q[3];   // continuous-kernel sums inside the width: sum d.mu, sum mu
+    __device__ void init() {
+        for (int i = 0; i < NO; ++i) { s[i] = 0.f; c[i] = 0.f; lp[i] = 0.f; fp[i] = 0.f; }
+        P = 0.f; Q[0] = Q[1] = Q[2] = 0.f;
+    }
+    // one interaction: d = x - y (FP32), mu / e / q of the source
+    __device__ void add(float* dst, const float* d, float r2, bool in, const float* mu,
+                        const float* e, const float* q) {
+        const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+        const float ir2 = ir * ir;
+        const float ir3 = ir2 * ir;
+        const float dmu = fmaf(d[0], mu[0], fmaf(d[1], mu[1], d[2] * mu[2]));
+        if constexpr (VOP == VOP_A) {
+            dst[0] = fmaf(dmu, ir3, dst[0]);
+        } else if constexpr (VOP == VOP_G) {
+            const float u = 3.f * dmu * (ir3 * ir2);
+            for (int k = 0; k < 3; ++k) dst[k] = fmaf(-u, d[k], fmaf(mu[k], ir3, dst[k]));
+        } else {
+            const float t = dmu * ir3;
+            for (int k = 0; k < 3; ++k) dst[k] = fmaf(e[k], t, fmaf(q[k], ir, dst[k]));
+        }
+        if constexpr (CONT) {
+            if (in) {
+                P += dmu;
+                for (int k = 0; k < 3; ++k) Q[k] += mu[k];
+            }
+        }
+    }
+    __device__ void flush_leaf() {
+        for (int i = 0; i < NO; ++i) {
+            neumaier(s[i], c[i], lp[i]);
+            neumaier(s[i], c[i], fp[i]);
+            lp[i] = 0.f;
+            fp[i] = 0.f;
+        }
+    }
+};
+
+template <int VOP, bool CONT>
+struct VAcc64 {
+    static constexpr int NO = (VOP == VOP_A) ? 1 : 3;
+    double s[NO];
+    double P, Q[3];
+    __device__ void init() {
+        for (int i = 0; i < NO; ++i) s[i] = 0.0;
+        P = 0.0; Q[0] = Q[1] = Q[2] = 0.0;
+    }
+    __device__ void add(const double* d, double w, const double* mu, double an) {
+        const double r2 = dist2_rn(d[0], d[1], d[2]);
+        const double r = sqrt(r2);
+        const double dmu = d[0] * mu[0] + d[1] * mu[1] + d[2] * mu[2];
+        if (r >= w) {
+            const double ir = 1.0 / r;
+            const double ir3 = ir * ir * ir;
+            if constexpr (VOP == VOP_A) {
+                s[0] += dmu * ir3;
+            } else if constexpr (VOP == VOP_G) {
+                const double ir5 = ir3 * ir * ir;
+                for (int k = 0; k < 3; ++k) s[k] += mu[k] * ir3 - 3.0 * d[k] * dmu * ir5;
+            } else {
+                const double t = dmu * ir3;
+                const double h = 0.5 * an * ir;
+                for (int k = 0; k < 3; ++k) s[k] += exp(0.5 * an * d[k]) * (t + h * mu[k]);
+            }
+        } else if constexpr (CONT) {
+            P += dmu;
+            for (int k = 0; k < 3; ++k) Q[k] += mu[k];
+        }
+    }
+};
+
+template <int P, int VOP, int QM, bool CONT, bool CNT>
+__global__ __launch_bounds__(kThreads) void k_trav_v(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave =
+        xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t idx = wave * 64 + lane;
+    double x[3], w;
+    int64_t oi;
+    const bool valid = load_query<QM>(p, pts64, idx, x, w, oi);
+    if (__ballot(valid) == 0) return;
+
+    int32_t resume = valid ? p.root : INT_MAX;
+    int64_t nfar = 0, nnear = 0, nvisit = 0;
+
+    if constexpr (P == CAGR_PREC_FP32C) {
+        const NodeV32* __restrict__ nodes = (const NodeV32*)nodes_;
+        const SrcV32* __restrict__ src = (const SrcV32*)src_;
+        float xf[3];
+        if constexpr (QM == QM_SELF) {
+            const int64_t s = valid ? idx : 0;
+            xf[0] = pts32[3 * s]; xf[1] = pts32[3 * s + 1]; xf[2] = pts32[3 * s + 2];
+        } else {
+            xf[0] = (float)x[0]; xf[1] = (float)x[1]; xf[2] = (float)x[2];
+        }
+        const float wf = (float)w;
+        const float w2 = wf * wf;
+        VAcc32<VOP, CONT> acc;
+        acc.init();
+        for (int32_t n = p.root; n < p.stop;) {
+            n = __builtin_amdgcn_readfirstlane(n);
+            const TopoRec t = topo[n];
+            const NodeV32 nr = nodes[n];
+            const bool act = resume <= n;
+            const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
+            const double d2 = dist2_rn(e0, e1, e2);
+            const bool far = d2 > t.thr2;
+            if constexpr (CNT) nvisit += act ? 1 : 0;
+            if (act && far) {
+                resume = t.escape;
+                if constexpr (CNT) {
+                    ++nfar;
+                } else {
+                    const float d[3] = {(float)e0, (float)e1, (float)e2};
+                    const float r2 = (float)d2;
+                    const bool in = inside_fp32(r2, w2, e0, e1, e2, w);
+                    acc.add(acc.fp, d, r2, in, nr.m, nr.e, nr.q);
+                }
+            }
+            const bool nearl = act && !far;
+            if (topo_leaf(t)) {
+                if (nearl) {
+                    resume = t.escape;
+                    if constexpr (CNT) {
+                        nnear += t.end - t.begin;
+                    } else {
+                        for (int32_t j = t.begin; j < t.end; ++j) {
+                            const SrcV32 sr = src[j];
+                            const float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
+                            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                            const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
+                                                        x[1] - (double)sr.y[1],
+                                                        x[2] - (double)sr.y[2], w);
+                            acc.add(acc.lp, d, r2, in, sr.mu, sr.e, sr.q);
+                        }
+                        acc.flush_leaf();
+                    }
+                }
+                n = t.escape;
+            } else {
+                n = __ballot(nearl) != 0 ? n + 1 : t.escape;
+            }
+        }
+        if constexpr (!CNT) {
+            acc.flush_leaf();
+            if (valid) {
+                double res[3];
+                for (int k = 0; k < acc.NO; ++k) res[k] = (double)acc.s[k] + (double)acc.c[k];
+                if constexpr (VOP == VOP_A) {
+                    double v = res[0];
+                    if constexpr (CONT) v += (double)acc.P / (w * w * w);
+                    store_out(p, oi, p.sign * v);
+                } else if constexpr (VOP == VOP_G) {
+                    for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, res[k]);
+                } else {
+                    double o[3];
+                    const double ew = CONT ? exp(p.an * w) : 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        o[k] = exp(0.5 * p.an * x[k]) * res[k];
+                        if constexpr (CONT)
+                            o[k] += ew * ((double)acc.P / (w * w * w) + 0.5 * p.an * (double)acc.Q[k] / w);
+                    }
+                    if constexpr (VOP == VOP_A3) {
+                        store_out(p, oi, ((o[0] + o[1]) + o[2]) / 3.0);
+                    } else if (p.kdir >= 0) {
+                        store_out(p, oi, sel3(o, p.kdir));
+                    } else {
+                        for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, o[k]);
+                    }
+                }
+            }
+        }
+    } else {
+        const NodeV64* __restrict__ nodes = (const NodeV64*)nodes_;
+        const SrcV64* __restrict__ src = (const SrcV64*)src_;
+        VAcc64<VOP, CONT> acc;
+        acc.init();
+        for (int32_t n = p.root; n < p.stop;) {
+            n = __builtin_amdgcn_readfirstlane(n);
+            const TopoRec t = topo[n];
+            const NodeV64 nr = nodes[n];
+            const bool act = resume <= n;
+            const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
+            const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
+            if constexpr (CNT) nvisit += act ? 1 : 0;
+            if (act && far) {
+                resume = t.escape;
+                if constexpr (CNT) ++nfar;
+                else acc.add(d, w, nr.m, p.an);
+            }
+            const bool nearl = act && !far;
+            if (topo_leaf(t)) {
+                if (nearl) {
+                    resume = t.escape;
+                    if constexpr (CNT) {
+                        nnear += t.end - t.begin;
+                    } else {
+                        for (int32_t j = t.begin; j < t.end; ++j) {
+                            const SrcV64 sr = src[j];
+                            const double dd[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
+                            acc.add(dd, w, sr.mu, p.an);
+                        }
+                    }
+                }
+                n = t.escape;
+            } else {
+                n = __ballot(nearl) != 0 ? n + 1 : t.escape;
+            }
+        }
+        if constexpr (!CNT) {
+            if (valid) {
+                if constexpr (VOP == VOP_A) {
+                    double v = acc.s[0];
+                    if constexpr (CONT) v += acc.P / (w * w * w);
+                    store_out(p, oi, p.sign * v);
+                } else if constexpr (VOP == VOP_G) {
+                    for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, acc.s[k]);
+                } else {
+                    double o[3];
+                    for (int k = 0; k < 3; ++k) {
+                        o[k] = acc.s[k];
+                        if constexpr (CONT)
+                            o[k] += exp(p.an * w) * (acc.P / (w * w * w) + 0.5 * p.an * acc.Q[k] / w);
+                    }
+                    if constexpr (VOP == VOP_A3) {
+                        store_out(p, oi, ((o[0] + o[1]) + o[2]) / 3.0);
+                    } else if (p.kdir >= 0) {
+                        store_out(p, oi, sel3(o, p.kdir));
+                    } else {
+                        for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, o[k]);
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (CNT) {
+        if (valid) {
+            if (p.cnt_far) p.cnt_far[oi] = nfar;
+            if (p.cnt_near) p.cnt_near[oi] = nnear;
+            if (p.cnt_visit) p.cnt_visit[oi] = nvisit;
+        }
+    }
+}
+
+// ============================================================================
+// scalar family: AT (an = 0), A2Tk (K = 1, direction kdir), sum_k A2Tk (K = 3)
+// ============================================================================
+template <int P, int K, int QM, bool CNT>
+__global__ __launch_bounds__(kThreads) void k_trav_s(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave =
+        xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t idx = wave * 64 + lane;
+    double x[3], w;
+    int64_t oi;
+    const bool valid = load_query<QM>(p, pts64, idx, x, w, oi);
+    if (__ballot(valid) == 0) return;
+
+    int32_t resume[K];
+    for (int k = 0; k < K; ++k) resume[k] = valid ? p.root : INT_MAX;
+    int64_t nfar = 0, nnear = 0, nvisit = 0;
+    const double han = 0.5 * p.an;
+
+    if constexpr (P == CAGR_PREC_FP32C) {
+        const NodeS32* __restrict__ nodes = (const NodeS32*)nodes_;
+        const SrcS32<K>* __restrict__ src = (const SrcS32<K>*)src_;
+        float xf[3];
+        if constexpr (QM == QM_SELF) {
+            const int64_t s = valid ? idx : 0;
+            xf[0] = pts32[3 * s]; xf[1] = pts32[3 * s + 1]; xf[2] = pts32[3 * s + 2];
+        } else {
+            xf[0] = (float)x[0]; xf[1] = (float)x[1]; xf[2] = (float)x[2];
+        }
+        float eq[K];   // exp(An * x_dir / 2) of each set's direction
+        for (int k = 0; k < K; ++k) eq[k] = (float)exp(han * sel3(x, p.kdir + k));
+        const float hanf = (float)han;
+        const float wf = (float)w;
+        const float w2 = wf * wf;
+        float s[3] = {0.f, 0.f, 0.f}, c[3] = {0.f, 0.f, 0.f}, lp[3] = {0.f, 0.f, 0.f},
+              fp[3] = {0.f, 0.f, 0.f};
+        for (int32_t n = p.root; n < p.stop;) {
+            n = __builtin_amdgcn_readfirstlane(n);
+            const TopoRec t = topo[n];
+            bool nearl[K];
+            bool any_near = false;
+            for (int k = 0; k < K; ++k) {
+                const NodeS32 nr = nodes[(int64_t)n * K + k];
+                const bool act = resume[k] <= n;
+                const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
+                const double d2 = dist2_rn(e0, e1, e2);
+                const bool far = d2 > t.thr2;
+                if constexpr (CNT) nvisit += act ? 1 : 0;
+                if (act && far) {
+                    resume[k] = t.escape;
+                    if constexpr (CNT) {
+                        ++nfar;
+                    } else {
+                        const float d[3] = {(float)e0, (float)e1, (float)e2};
+                        const float r2 = (float)d2;
+                        const bool in = inside_fp32(r2, w2, e0, e1, e2, w);
+                        const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+                        const float g = eq[k] * nr.e * nr.s;
+                        const float gi3 = g * (ir * ir * ir);
+                        const float hg = hanf * g * ir;
+                        const int dk = p.kdir + k;
+                        for (int cc = 0; cc < 3; ++cc)
+                            fp[cc] = fmaf(gi3, d[cc], fp[cc]) + (cc == dk ? hg : 0.f);
+                    }
+                }
+                nearl[k] = act && !far;
+                any_near |= nearl[k];
+            }
+            if (topo_leaf(t)) {
+                if (any_near) {
+                    for (int k = 0; k < K; ++k)
+                        if (nearl[k]) resume[k] = t.escape;
+                    if constexpr (CNT) {
+                        for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
+                    } else {
+                        float mk[K];
+                        for (int k = 0; k < K; ++k) mk[k] = nearl[k] ? eq[k] : 0.f;
+                        for (int32_t j = t.begin; j < t.end; ++j) {
+                            const SrcS32<K> sr = src[j];
+                            const float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
+                            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                            const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
+                                                        x[1] - (double)sr.y[1],
+                                                        x[2] - (double)sr.y[2], w);
+                            const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+                            const float ir3 = ir * ir * ir;
+                            float gsum = 0.f;
+                            float g[K];
+                            for (int k = 0; k < K; ++k) { g[k] = mk[k] * sr.a[k]; gsum += g[k]; }
+                            const float gi3 = gsum * ir3;
+                            for (int cc = 0; cc < 3; ++cc) lp[cc] = fmaf(gi3, d[cc], lp[cc]);
+                            const float hir = hanf * ir;
+                            if constexpr (K == 3) {
+                                for (int k = 0; k < 3; ++k) lp[k] = fmaf(g[k], hir, lp[k]);
+                            } else {
+                                const float v = g[0] * hir;
+                                for (int cc = 0; cc < 3; ++cc) lp[cc] += (cc == p.kdir) ? v : 0.f;
+                            }
+                        }
+                        for (int cc = 0; cc < 3; ++cc) {
+                            neumaier(s[cc], c[cc], lp[cc]);
+                            neumaier(s[cc], c[cc], fp[cc]);
+                            lp[cc] = 0.f;
+                            fp[cc] = 0.f;
+                        }
+                    }
+                }
+                n = t.escape;
+            } else {
+                n = __ballot(any_near) != 0 ? n + 1 : t.escape;
+            }
+        }
+        if constexpr (!CNT) {
+            if (valid)
+                for (int cc = 0; cc < 3; ++cc) {
+                    neumaier(s[cc], c[cc], fp[cc]);
+                    store_out(p, 3 * oi + cc, (double)s[cc] + (double)c[cc]);
+                }
+        }
+    } else {
+        const NodeS64* __restrict__ nodes = (const NodeS64*)nodes_;
+        const SrcS64<K>* __restrict__ src = (const SrcS64<K>*)src_;
+        double acc[3] = {0.0, 0.0, 0.0};
+        // one A2T_dir interaction (AT when an = 0) of scalar sv at d
+        auto add = [&](const double* d, double sv, int dir) {
+            const double r2 = dist2_rn(d[0], d[1], d[2]);
+            const double r = sqrt(r2);
+            if (r >= w) {
+                const double ir = 1.0 / r;
+                const double g = exp(han * sel3(d, dir)) * sv;
+                const double gi3 = g * (ir * ir * ir);
+                const double hg = han * g * ir;
+                for (int cc = 0; cc < 3; ++cc) acc[cc] += gi3 * d[cc] + (cc == dir ? hg : 0.0);
+            }
+        };
+        for (int32_t n = p.root; n < p.stop;) {
+            n = __builtin_amdgcn_readfirstlane(n);
+            const TopoRec t = topo[n];
+            bool nearl[K];
+            bool any_near = false;
+            for (int k = 0; k < K; ++k) {
+                const NodeS64 nr = nodes[(int64_t)n * K + k];
+                const bool act = resume[k] <= n;
+                const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
+                const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
+                if constexpr (CNT) nvisit += act ? 1 : 0;
+                if (act && far) {
+                    resume[k] = t.escape;
+                    if constexpr (CNT) ++nfar;
+                    else add(d, nr.s, p.kdir + k);
+                }
+                nearl[k] = act && !far;
+                any_near |= nearl[k];
+            }
+            if (topo_leaf(t)) {
+                if (any_near) {
+                    for (int k = 0; k < K; ++k)
+                        if (nearl[k]) resume[k] = t.escape;
+                    if constexpr (CNT) {
+                        for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
+                    } else {
+                        for (int32_t j = t.begin; j < t.end; ++j) {
+                            const SrcS64<K> sr = src[j];
+                            const double d[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
+                            for (int k = 0; k < K; ++k)
+                                if (nearl[k]) add(d, sr.s[k], p.kdir + k);
+                        }
+                    }
+                }
+                n = t.escape;
+            } else {
+                n = __ballot(any_near) != 0 ? n + 1 : t.escape;
+            }
+        }
+        if constexpr (!CNT) {
+            if (valid)
+                for (int cc = 0; cc < 3; ++cc) store_out(p, 3 * oi + cc, acc[cc]);
+        }
+    }
+    if constexpr (CNT) {
+        if (valid) {
+            if (p.cnt_far) p.cnt_far[oi] = nfar;
+            if (p.cnt_near) p.cnt_near[oi] = nnear;
+            if (p.cnt_visit) p.cnt_visit[oi] = nvisit;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int P, int VOP, int QM, bool CNT>
+int launch_v(const TP& tp, cagr_tree* t, int continuous, int64_t blocks, hipStream_t st) {
+    if (continuous)
+        k_trav_v<P, VOP, QM, true, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src,
+                                                                    t->pts64, t->pts32);
+    else
+        k_trav_v<P, VOP, QM, false, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src,
+                                                                     t->pts64, t->pts32);
+    CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+
+template <int P, int QM, bool CNT>
+int launch_vop(const TP& tp, cagr_tree* t, int vop, int continuous, int64_t blocks,
+               hipStream_t st) {
+    switch (vop) {
+        case VOP_A: return launch_v<P, VOP_A, QM, CNT>(tp, t, continuous, blocks, st);
+        case VOP_G: return launch_v<P, VOP_G, QM, CNT>(tp, t, 0, blocks, st);
+        case VOP_A2: return launch_v<P, VOP_A2, QM, CNT>(tp, t, continuous, blocks, st);
+        default: return launch_v<P, VOP_A3, QM, CNT>(tp, t, continuous, blocks, st);
+    }
+}
+
+template <int P, int K, int QM, bool CNT>
+int launch_s(const TP& tp, cagr_tree* t, int64_t blocks, hipStream_t st) {
+    k_trav_s<P, K, QM, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src, t->pts64,
+                                                         t->pts32);
+    CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+
+template <int P, int QM, bool CNT>
+int launch_all(const TP& tp, cagr_tree* t, int op, int continuous, int64_t blocks,
+               hipStream_t st) {
+    switch (op) {
+        case CAGR_OP_A:
+        case CAGR_OP_A_CPU: return launch_vop<P, QM, CNT>(tp, t, VOP_A, continuous, blocks, st);
+        case CAGR_OP_G: return launch_vop<P, QM, CNT>(tp, t, VOP_G, 0, blocks, st);
+        case CAGR_OP_A21:
+        case CAGR_OP_A22:
+        case CAGR_OP_A23:
+        case CAGR_OP_A2: return launch_vop<P, QM, CNT>(tp, t, VOP_A2, continuous, blocks, st);
+        case CAGR_OP_A3: return launch_vop<P, QM, CNT>(tp, t, VOP_A3, continuous, blocks, st);
+        case CAGR_OP_AT:
+        case CAGR_OP_A2T1:
+        case CAGR_OP_A2T2:
+        case CAGR_OP_A2T3:
+            if constexpr (QM == QM_GRID) return CAGR_ERR_ARG;
+            else return launch_s<P, 1, QM, CNT>(tp, t, blocks, st);
+        case CAGR_OP_A2T_SUM:
+            if constexpr (QM == QM_GRID) return CAGR_ERR_ARG;
+            else return launch_s<P, 3, QM, CNT>(tp, t, blocks, st);
+    }
+    set_error("unknown op");
+    return CAGR_ERR_ARG;
+}
+
+}  // namespace
+
+int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuous, double an,
+             void* out, int odtype, int64_t* cnt_far, int64_t* cnt_near, int64_t* cnt_visit,
+             hipStream_t st) {
+    TP tp{};
+    tp.root = 0;
+    tp.stop = (int32_t)t->m;
+    tp.perm = t->perm;
+    tp.q = q.q;
+    tp.qdtype = q.qdtype;
+    tp.qorder = q.qorder;
+    tp.widths = q.widths;
+    tp.wdtype = q.wdtype;
+    tp.width = q.width;
+    tp.out = out;
+    tp.odtype = odtype;
+    tp.an = an;
+    tp.sign = (op == CAGR_OP_A_CPU) ? -1.0 : 1.0;
+    tp.kdir = -1;
+    if (op >= CAGR_OP_A21 && op <= CAGR_OP_A23) tp.kdir = op - CAGR_OP_A21;
+    if (op >= CAGR_OP_A2T1 && op <= CAGR_OP_A2T3) tp.kdir = op - CAGR_OP_A2T1;
+    if (op == CAGR_OP_AT || op == CAGR_OP_A2T_SUM) tp.kdir = 0;
+    if (op == CAGR_OP_AT) tp.an = 0.0;
+    tp.cnt_far = cnt_far;
+    tp.cnt_near = cnt_near;
+    tp.cnt_visit = cnt_visit;
+    int64_t lanes;
+    if (q.mode == QM_GRID) {
+        for (int d = 0; d < 3; ++d) {
+            tp.glo[d] = q.glo[d];
+            tp.gstep[d] = q.gstep[d];
+            tp.gdim[d] = q.gdim[d];
+            tp.gblk[d] = (q.gdim[d] + 3) / 4;
+        }
+        lanes = tp.gblk[0] * tp.gblk[1] * tp.gblk[2] * 64;
+    } else {
+        lanes = q.nq;
+    }
+    tp.nq = lanes;
+    if (lanes == 0) return CAGR_OK;
+    const int64_t waves = (lanes + 63) / 64;
+    const int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    const bool cnt = cnt_far || cnt_near || cnt_visit;
+    const bool p64 = precision == CAGR_PREC_FP64;
+    int rc;
+#define CAGR_LAUNCH_QM(QMV)                                                                   \
+    if (cnt) rc = launch_all<CAGR_PREC_FP64, QMV, true>(tp, t, op, continuous, blocks, st);    \
+    else if (p64) rc = launch_all<CAGR_PREC_FP64, QMV, false>(tp, t, op, continuous, blocks, st); \
+    else rc = launch_all<CAGR_PREC_FP32C, QMV, false>(tp, t, op, continuous, blocks, st);
+    if (q.mode == QM_SELF) {
+        CAGR_LAUNCH_QM(QM_SELF)
+    } else if (q.mode == QM_EXPLICIT) {
+        CAGR_LAUNCH_QM(QM_EXPLICIT)
+    } else {
+        CAGR_LAUNCH_QM(QM_GRID)
+    }
+#undef CAGR_LAUNCH_QM
+    return rc;
+}
+
+}  // namespace cagr

@@ -1,0 +1,355 @@
+// Octree build on the GPU (gfx950): exact Morton keys, a stable LSD radix
+// sort, and the preorder topology from the sorted keys.  The per-thread
+// bodies live in tree_logic.h (shared with the CPU replay test); this file
+// holds the kernels, the radix sort and the host orchestration.
+//
+// Reference behaviour reproduced: wn_treecode_cpu_treeutils.cpp:90-258 and
+// wn_treecode_cpu_torch_interface.cpp:41-102 (root box, leaf rule, preorder
+// ids, ascending point lists).
+#include "common.h"
+#include "tree_export.h"
+#include <algorithm>
+#include <functional>
+
+namespace cagr {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline int64_t blocks_for(int64_t n, int per = kThreads) { return (n + per - 1) / per; }
+
+// ---------------------------------------------------------------- keys
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_keys(const T* __restrict__ pts, int64_t n,
+                                                   uint64_t* __restrict__ keys,
+                                                   int32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = morton_key(cell_index(pts[3 * i]), cell_index(pts[3 * i + 1]),
+                         cell_index(pts[3 * i + 2]));
+    idx[i] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------- radix sort
+// LSD, 8-bit digits, tiles of 2048 keys per 256-thread block.  Stability:
+// items of a tile are ranked in (round, thread) order; within a wave by a
+// 64-lane digit match built from 8 ballots; waves by an LDS prefix; tiles by
+// the bin-major scan of the per-tile histograms.
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kThreads * kRsItems;
+
+__global__ __launch_bounds__(kThreads) void k_rs_hist(const uint64_t* __restrict__ keys,
+                                                      int64_t n, int shift,
+                                                      uint32_t* __restrict__ hist,
+                                                      int64_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        const int64_t i = base + r * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of a[0..len) in one block of 1024 threads
+template <typename U>
+__global__ __launch_bounds__(1024) void k_scan_excl(U* __restrict__ a, int64_t len) {
+    __shared__ U part[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (len + 1023) / 1024;
+    const int64_t lo = t * chunk;
+    const int64_t hi = lo + chunk < len ? lo + chunk : len;
+    U s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += a[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const U v = t >= off ? part[t - off] : (U)0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    U run = part[t] - s;
+    for (int64_t i = lo; i < hi; ++i) {
+        const U v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_scatter(
+    const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
+    const uint32_t* __restrict__ offs, int64_t nblocks) {
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t gbase[256];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    run[t] = 0;
+    gbase[t] = offs[(int64_t)t * nblocks + blockIdx.x];
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        wcnt[0][t] = 0;
+        wcnt[1][t] = 0;
+        wcnt[2][t] = 0;
+        wcnt[3][t] = 0;
+        __syncthreads();
+        const int64_t i = base + r * kThreads + t;
+        const bool valid = i < n;
+        const uint64_t key = valid ? kin[i] : 0ull;
+        const int32_t val = valid ? vin[i] : 0;
+        const uint32_t dig = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bb = __ballot((dig >> b) & 1u);
+            peers &= ((dig >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lanes_below);
+        if (valid && rank == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t acc = run[t];
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t c = wcnt[k][t];
+            wcnt[k][t] = acc;
+            acc += c;
+        }
+        run[t] = acc;
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = gbase[dig] + wcnt[w][dig] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- topology
+__global__ __launch_bounds__(kThreads) void k_chain(const uint64_t* __restrict__ keys, int64_t n,
+                                                    int S, int max_depth, int8_t* __restrict__ ld,
+                                                    int8_t* __restrict__ sd,
+                                                    int32_t* __restrict__ cnt) {
+    const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b < n) chain_body(b, keys, n, S, max_depth, ld, sd, cnt);
+    if (b == n) cnt[n] = 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_nodes(const uint64_t* __restrict__ keys, int64_t n,
+                                                    const int8_t* __restrict__ ld,
+                                                    const int8_t* __restrict__ sd,
+                                                    const int32_t* __restrict__ off,
+                                                    double open_factor,
+                                                    TopoRec* __restrict__ topo) {
+    const int64_t b = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b < n) node_body(b, keys, n, ld, sd, off, open_factor, topo);
+}
+
+__global__ __launch_bounds__(kThreads) void k_leaf_sort(const TopoRec* __restrict__ topo,
+                                                        int64_t m, int32_t* __restrict__ perm) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= m) return;
+    const TopoRec t = topo[i];
+    if (topo_leaf(t)) leaf_sort_body(t, perm);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gather_points(const T* __restrict__ pts,
+                                                            const int32_t* __restrict__ perm,
+                                                            int64_t n, double* __restrict__ p64,
+                                                            float* __restrict__ p32) {
+    const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (s >= n) return;
+    const int64_t j = perm[s];
+    for (int d = 0; d < 3; ++d) {
+        const T v = pts[3 * j + d];
+        p64[3 * s + d] = (double)v;
+        p32[3 * s + d] = (float)v;
+    }
+}
+
+// group of a node in the upward order: 0 = leaf, 1 + (14 - depth) = internal
+__device__ inline int up_group(const TopoRec& t) {
+    return topo_leaf(t) ? 0 : 1 + (kLevels - 1 - topo_depth(t));
+}
+constexpr int kGroups = 1 + kLevels;
+
+__global__ __launch_bounds__(kThreads) void k_group_count(const TopoRec* __restrict__ topo,
+                                                          int64_t m, int32_t* __restrict__ cnt,
+                                                          int32_t* __restrict__ max_depth) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= m) return;
+    const TopoRec t = topo[i];
+    atomicAdd(&cnt[up_group(t)], 1);
+    if (topo_leaf(t)) atomicMax(max_depth, topo_depth(t));
+}
+
+__global__ __launch_bounds__(kThreads) void k_group_fill(const TopoRec* __restrict__ topo,
+                                                         int64_t m, int32_t* __restrict__ cursor,
+                                                         int32_t* __restrict__ order) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < m) order[atomicAdd(&cursor[up_group(topo[i])], 1)] = (int32_t)i;
+}
+
+}  // namespace
+
+int64_t sort_hist_len(int64_t n) { return 256 * std::max<int64_t>(1, (n + kRsTile - 1) / kRsTile); }
+
+// Sorts (keys, vals) by the low `bits` bits of keys, stably.  The result is
+// left in keys/vals; *_tmp and hist (sort_hist_len(n) entries) are scratch.
+int sort_pairs_u64(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp,
+                   uint32_t* hist, int64_t n, int bits, hipStream_t st) {
+    if (n <= 1) return CAGR_OK;
+    const int64_t nb = (n + kRsTile - 1) / kRsTile;
+    uint64_t* ka = keys;
+    uint64_t* kb = keys_tmp;
+    int32_t* va = vals;
+    int32_t* vb = vals_tmp;
+    int passes = 0;
+    for (int shift = 0; shift < bits; shift += 8, ++passes) {
+        k_rs_hist<<<nb, kThreads, 0, st>>>(ka, n, shift, hist, nb);
+        CAGR_LAUNCHED();
+        k_scan_excl<uint32_t><<<1, 1024, 0, st>>>(hist, 256 * nb);
+        CAGR_LAUNCHED();
+        k_rs_scatter<<<nb, kThreads, 0, st>>>(ka, va, kb, vb, n, shift, hist, nb);
+        CAGR_LAUNCHED();
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (passes & 1) {
+        CAGR_HIP(hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+        CAGR_HIP(hipMemcpyAsync(vals, va, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    }
+    return CAGR_OK;
+}
+
+// Morton order of n arbitrary points (keys on the tree's root box, same cell
+// index as the build): order[i] = index of the i-th point in Morton order.
+size_t morton_order_scratch(int64_t n) {
+    return 2 * n * sizeof(uint64_t) + n * sizeof(int32_t) + sort_hist_len(n) * sizeof(uint32_t) + 256;
+}
+
+int morton_order(const void* pts, int dtype, int64_t n, int32_t* order, void* scratch,
+                 hipStream_t st) {
+    if (n == 0) return CAGR_OK;
+    char* p = (char*)scratch;
+    uint64_t* keys = (uint64_t*)p;
+    p += n * sizeof(uint64_t);
+    uint64_t* ktmp = (uint64_t*)p;
+    p += n * sizeof(uint64_t);
+    int32_t* vtmp = (int32_t*)p;
+    p += n * sizeof(int32_t);
+    p = (char*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    uint32_t* hist = (uint32_t*)p;
+    const int64_t nb = blocks_for(n);
+    if (dtype == CAGR_F32)
+        k_keys<float><<<nb, kThreads, 0, st>>>((const float*)pts, n, keys, order);
+    else
+        k_keys<double><<<nb, kThreads, 0, st>>>((const double*)pts, n, keys, order);
+    CAGR_LAUNCHED();
+    return sort_pairs_u64(keys, order, ktmp, vtmp, hist, n, kKeyBits, st);
+}
+
+int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_depth,
+               int leaf_size, double theta, hipStream_t st) {
+    t->n = n;
+    t->max_depth = max_depth;
+    t->leaf_size = leaf_size;
+    t->theta = theta;
+    t->open_factor = 2.0 / theta;
+    t->point_dtype = dtype;
+    CAGR_HIP(hipGetDevice(&t->device));
+    if (n == 0) {   // the reference builds no node at all for an empty cloud
+        t->m = 0;
+        t->group_off.assign(kGroups + 1, 0);
+        return CAGR_OK;
+    }
+
+    // scratch for the build (freed at the end)
+    uint64_t *ktmp = nullptr;
+    int32_t *vtmp = nullptr, *cnt = nullptr, *gcnt = nullptr;
+    uint32_t* hist = nullptr;
+    int8_t *ld = nullptr, *sd = nullptr;
+    auto free_scratch = [&]() {
+        (void)hipFree(ktmp); (void)hipFree(vtmp); (void)hipFree(cnt); (void)hipFree(gcnt); (void)hipFree(hist);
+        (void)hipFree(ld); (void)hipFree(sd);
+    };
+    struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{free_scratch};
+
+    CAGR_HIP(hipMalloc(&t->keys, n * sizeof(uint64_t)));
+    CAGR_HIP(hipMalloc(&t->perm, n * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&t->pts64, n * 3 * sizeof(double)));
+    CAGR_HIP(hipMalloc(&t->pts32, n * 3 * sizeof(float)));
+    CAGR_HIP(hipMalloc(&ktmp, n * sizeof(uint64_t)));
+    CAGR_HIP(hipMalloc(&vtmp, n * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&hist, sort_hist_len(n) * sizeof(uint32_t)));
+    CAGR_HIP(hipMalloc(&cnt, (n + 1) * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&ld, n));
+    CAGR_HIP(hipMalloc(&sd, n));
+    CAGR_HIP(hipMalloc(&gcnt, (2 * kGroups + 1) * sizeof(int32_t)));
+
+    const int64_t nb = blocks_for(n);
+    if (dtype == CAGR_F32)
+        k_keys<float><<<nb, kThreads, 0, st>>>((const float*)points, n, t->keys, t->perm);
+    else
+        k_keys<double><<<nb, kThreads, 0, st>>>((const double*)points, n, t->keys, t->perm);
+    CAGR_LAUNCHED();
+    int rc = sort_pairs_u64(t->keys, t->perm, ktmp, vtmp, hist, n, kKeyBits, st);
+    if (rc) return rc;
+
+    k_chain<<<blocks_for(n + 1), kThreads, 0, st>>>(t->keys, n, leaf_size, max_depth, ld, sd, cnt);
+    CAGR_LAUNCHED();
+    k_scan_excl<int32_t><<<1, 1024, 0, st>>>(cnt, n + 1);
+    CAGR_LAUNCHED();
+    int32_t m32 = 0;
+    CAGR_HIP(hipMemcpyAsync(&m32, cnt + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    const int64_t m = m32;
+    t->m = m;
+    CAGR_REQUIRE(m > 0, CAGR_ERR_STATE, "tree build produced no nodes");
+
+    CAGR_HIP(hipMalloc(&t->topo, m * sizeof(TopoRec)));
+    CAGR_HIP(hipMalloc(&t->order, m * sizeof(int32_t)));
+    k_nodes<<<nb, kThreads, 0, st>>>(t->keys, n, ld, sd, cnt, t->open_factor, t->topo);
+    CAGR_LAUNCHED();
+    k_leaf_sort<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->perm);
+    CAGR_LAUNCHED();
+    if (dtype == CAGR_F32)
+        k_gather_points<float><<<nb, kThreads, 0, st>>>((const float*)points, t->perm, n, t->pts64, t->pts32);
+    else
+        k_gather_points<double><<<nb, kThreads, 0, st>>>((const double*)points, t->perm, n, t->pts64, t->pts32);
+    CAGR_LAUNCHED();
+
+    // upward order: all leaves, then internal nodes from depth 14 up to 0
+    CAGR_HIP(hipMemsetAsync(gcnt, 0, (2 * kGroups + 1) * sizeof(int32_t), st));
+    k_group_count<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, gcnt, gcnt + 2 * kGroups);
+    CAGR_LAUNCHED();
+    int32_t hc[kGroups], hdepth = 0;
+    CAGR_HIP(hipMemcpyAsync(hc, gcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipMemcpyAsync(&hdepth, gcnt + 2 * kGroups, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    t->group_off.assign(kGroups + 1, 0);
+    for (int g = 0; g < kGroups; ++g) t->group_off[g + 1] = t->group_off[g] + hc[g];
+    int32_t cur[kGroups];
+    for (int g = 0; g < kGroups; ++g) cur[g] = (int32_t)t->group_off[g];
+    CAGR_HIP(hipMemcpyAsync(gcnt + kGroups, cur, sizeof(cur), hipMemcpyHostToDevice, st));
+    k_group_fill<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, gcnt + kGroups, t->order);
+    CAGR_LAUNCHED();
+    t->n_leaves = hc[0];
+    t->depth = hdepth;
+
+    // workspace sized for every operator family
+    CAGR_HIP(hipMalloc(&t->src, n * 64));
+    CAGR_HIP(hipMalloc(&t->nodes, m * 128));
+    CAGR_HIP(hipMalloc(&t->w64, 3 * m * sizeof(double)));
+    CAGR_HIP(hipMalloc(&t->r64, 9 * m * sizeof(double)));
+    CAGR_HIP(hipMalloc(&t->m64, 9 * m * sizeof(double)));
+    CAGR_HIP(hipStreamSynchronize(st));
+    return CAGR_OK;
+}
+
+}  // namespace cagr

@@ -1,0 +1,147 @@
+/*
+ * cagr_amd -- MI355X (gfx950) treecode Gauss-formula operator of CAGR.
+ *
+ * C ABI.  Plain pointers and sizes only; every array argument is a device
+ * pointer unless the comment says "host".  Every function returns a status
+ * (CAGR_OK = 0); on failure cagr_last_error() holds a message for the calling
+ * thread.  Work is enqueued on the given HIP stream (NULL = legacy default
+ * stream) and is asynchronous unless noted.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo root,
+ * CUDA-based/ext/wn_treecode/ abbreviated to wnt/):
+ *   wnt/wn_treecode_cpu/wn_treecode_cpu_torch_interface.cpp:104-118  build_tree
+ *   wnt/wn_treecode_func.py:23-27                                    tree H2D copy
+ *   wnt/wn_treecode_cuda/wn_treecode_cuda_torch_interface.cu:41-134  scatter_point_attrs_to_nodes
+ *   wnt/wn_treecode_cuda/wn_treecode_cuda_torch_interface.cu:137-777 multiply_by_{A,AT,A3,G,A21,A22,A23,A2T1,A2T2,A2T3}
+ *   CUDA-based/ext/gaussrecon_src/main_GaussRecon_cuda.cu:427-466     multiply_by_A3 grid query
+ */
+#ifndef CAGR_AMD_H
+#define CAGR_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CAGR_OK 0
+#define CAGR_ERR_ARG 1       /* invalid argument (shape, dtype, op, size) */
+#define CAGR_ERR_HIP 2       /* HIP runtime error */
+#define CAGR_ERR_STATE 3     /* handle used in a state that does not allow it */
+
+/* element types of caller arrays */
+#define CAGR_F32 0
+#define CAGR_F64 1
+
+/* precision of the operator arithmetic */
+#define CAGR_PREC_FP64 0     /* plain FP64 throughout (reference formula) */
+#define CAGR_PREC_FP32C 1    /* FP32 kernel sums, two-level compensated (Neumaier)
+                                accumulation; opening tests, cutoff ties and node
+                                moments in FP64 */
+
+/* operators.  in = attribute dim per point, out = output dim per query.  */
+#define CAGR_OP_A 0          /* in 3, out 1: sum (x-y).mu/r^3 (CUDA sign, cuda_kernels.cu:79-110) */
+#define CAGR_OP_A_CPU 1      /* in 3, out 1: negated A (CPU sign, cpu_kernels.cpp:77-103) */
+#define CAGR_OP_AT 2         /* in 1, out 3: sum (x-y) s/r^3 (cuda_kernels.cu:228-258) */
+#define CAGR_OP_G 3          /* in 3, out 3: sum mu/r^3 - 3 d (d.mu)/r^5 (cuda_kernels.cu:384-414) */
+#define CAGR_OP_A21 4        /* in 3, out 1: convection kernel, direction x (cuda_kernels.cu:112-150) */
+#define CAGR_OP_A22 5        /* ... direction y (cuda_kernels.cu:152-188) */
+#define CAGR_OP_A23 6        /* ... direction z (cuda_kernels.cu:190-224) */
+#define CAGR_OP_A2T1 7       /* in 1, out 3: reference transpose, direction x (cuda_kernels.cu:262-300) */
+#define CAGR_OP_A2T2 8       /* ... direction y (cuda_kernels.cu:302-339) */
+#define CAGR_OP_A2T3 9       /* ... direction z (cuda_kernels.cu:341-379) */
+#define CAGR_OP_A3 10        /* in 3, out 1: (A21+A22+A23)/3 (cuda_kernels.cu:694-763) */
+#define CAGR_OP_A2 11        /* in 3, out 3: (A21, A22, A23) from one upward pass + one
+                                traversal (the three reference calls fused) */
+#define CAGR_OP_A2T_SUM 12   /* in 3, out 3: A2T1(s[:,0]) + A2T2(s[:,1]) + A2T3(s[:,2]),
+                                each with its own node centres, one traversal
+                                (main_CAGR.py:109-111 pattern fused) */
+
+typedef struct cagr_tree cagr_tree;
+
+const char* cagr_last_error(void);
+int cagr_version(void);
+
+/* Octree over points [n,3] (dtype CAGR_F32/F64, original order, device).
+ * Replaces build_tree (wn_treecode_cpu_torch_interface.cpp:41-118) + the H2D
+ * copy of wn_treecode_func.py:23-27; the tree is built on the GPU.
+ * max_depth <= 15 (ALLOWED_MAX_DEPTH), 1 <= leaf_size <= 64 (the reference
+ * uses 1), theta in (0, 2] (the reference's TREECODE_THRESHOLD 2.0 is
+ * theta = 0.5): a node is far from a query iff dist^2(query, node centre) >
+ * (2/theta * half_w)^2.  Synchronises the stream (node count is read back).
+ * Allocates all device memory the handle needs; later calls on it allocate
+ * nothing except cagr_apply_queries with a larger query count than before. */
+int cagr_tree_build(const void* points, int64_t n, int dtype, int max_depth, int leaf_size,
+                    double theta, void* stream, cagr_tree** out);
+int cagr_tree_destroy(cagr_tree* tree);
+
+/* Host outputs: num_nodes M, num_leaves, depth of the deepest node, and the
+ * length of the reference node2point_index list. */
+int cagr_tree_info(const cagr_tree* tree, int64_t* num_nodes, int64_t* num_leaves,
+                   int64_t* depth, int64_t* list_len);
+
+/* The reference's serialised tree (wn_treecode_cpu_treeutils.cpp:202-258),
+ * written to HOST buffers: parent [M], children [M,8], is_leaf [M],
+ * half_w [M], num_points [M], node2point_index [list_len] (may be NULL),
+ * node2point_indexstart [M].  Synchronous. */
+int cagr_tree_export(const cagr_tree* tree, int64_t* parent, int64_t* children,
+                     uint8_t* is_leaf, double* half_w, int64_t* num_points,
+                     int64_t* node2point_index, int64_t* node2point_indexstart);
+
+/* Sorted-to-original point permutation [n] int32 (HOST).  Synchronous. */
+int cagr_tree_permutation(const cagr_tree* tree, int32_t* perm_host);
+
+/* Upward pass in FP64 (replaces scatter_point_attrs_to_nodes,
+ * wn_treecode_cuda_torch_interface.cu:41-134): attrs [n,attr_dim] (attr_dim 1
+ * or 3, original order), weights [n] or NULL for ||attr||_2
+ * (wn_treecode_func.py:131).  Outputs (device, FP64, node ids = preorder =
+ * the reference's): node_attrs [M,attr_dim], node_reppoints [M,3],
+ * node_weights [M]. */
+int cagr_upward(cagr_tree* tree, const void* attrs, int dtype, int attr_dim,
+                const void* weights, double* node_attrs, double* node_reppoints,
+                double* node_weights, void* stream);
+
+/* Operator with the tree's own points as queries (every forward_* of
+ * wn_treecode_func.py): upward pass with weights ||attr|| per set, then the
+ * traversal.  attrs [n,in] and widths [n] in `dtype`, original order; out
+ * [n,out] in `dtype`.  continuous selects the smoothed inner kernel of A, A_cpu,
+ * A2k, A2 and A3 (pybind default: false).  an is the convection strength
+ * (the reference's An = 1). */
+int cagr_apply(cagr_tree* tree, int op, int precision, const void* attrs, int dtype,
+               const void* widths, int continuous, double an, void* out, void* stream);
+
+/* As cagr_apply, but with the node centres taken from `weights` [n] (dtype)
+ * instead of ||attr||: the operator is then linear in attrs.  weights = NULL
+ * is cagr_apply. */
+int cagr_apply_weighted(cagr_tree* tree, int op, int precision, const void* attrs, int dtype,
+                        const void* weights, const void* widths, int continuous, double an,
+                        void* out, void* stream);
+
+/* Operator at external queries [nq,3] (qdtype) -- e.g. the GaussRecon sample
+ * isovalue pass (main_GaussRecon_cuda.cu:450-467).  widths [nq] (dtype) or
+ * NULL for the scalar `width`.  Queries are Morton-ordered internally for the
+ * traversal; out [nq,out] (dtype) is in the caller's order. */
+int cagr_apply_queries(cagr_tree* tree, int op, int precision, const void* attrs, int dtype,
+                       const void* queries, int qdtype, int64_t nq, const void* widths,
+                       double width, int continuous, double an, void* out, void* stream);
+
+/* Operator on the regular grid lo + i*step, i in [0,dims) per axis (HOST
+ * lo/step/dims), generated in-kernel; out [dims0*dims1*dims2] (dtype),
+ * row-major with axis 2 fastest.  op in {A, A_CPU, A21..A23, A3}; one scalar
+ * width.  Replaces the GaussRecon corner query (main_GaussRecon_cuda.cu:427-445)
+ * on a regular grid. */
+int cagr_grid_query(cagr_tree* tree, int op, int precision, const void* attrs, int dtype,
+                    const double* lo, const double* step, const int64_t* dims, double width,
+                    int continuous, double an, void* out, void* stream);
+
+/* Exact interaction counts of `op`'s traversal with the tree's own points as
+ * queries (int64 [n] each, original order; any may be NULL): far-node
+ * interactions, leaf-point interactions, nodes visited -- summed over the
+ * opening sets of the op. */
+int cagr_count(cagr_tree* tree, int op, const void* attrs, int dtype, const void* weights,
+               int64_t* n_far, int64_t* n_near, int64_t* n_visit, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CAGR_AMD_H */

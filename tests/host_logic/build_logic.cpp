@@ -1,0 +1,54 @@
+// CPU replay of the HIP tree build (test only).  Compiles the shared
+// per-element logic of cagr_amd/csrc/tree_logic.h with g++ and runs the same
+// sequence the kernels run -- keys, stable sort, chains, scan, node records,
+// leaf sorts -- serially, then writes the reference serialised layout so the
+// test can compare it with the oracle tree.
+#define CAGR_HD
+#include "../../cagr_amd/csrc/tree_logic.h"
+#include "../../cagr_amd/csrc/tree_export.h"
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+using namespace cagr;
+
+extern "C" int64_t hl_build(const double* pts, int64_t n, int max_depth, int S, double theta,
+                            int64_t* m_out, TopoRec* topo_out /* may be null */,
+                            int32_t* perm_out /* may be null */) {
+    std::vector<uint64_t> key(n);
+    for (int64_t i = 0; i < n; ++i)
+        key[i] = morton_key(cell_index(pts[3 * i]), cell_index(pts[3 * i + 1]),
+                            cell_index(pts[3 * i + 2]));
+    std::vector<int32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    std::vector<uint64_t> sk(n);
+    for (int64_t i = 0; i < n; ++i) sk[i] = key[perm[i]];
+    std::vector<int8_t> ld(n), sd(n);
+    std::vector<int32_t> cnt(n), off(n + 1, 0);
+    for (int64_t b = 0; b < n; ++b) chain_body(b, sk.data(), n, S, max_depth, ld.data(), sd.data(), cnt.data());
+    for (int64_t b = 0; b < n; ++b) off[b + 1] = off[b] + cnt[b];
+    const int64_t m = off[n];
+    std::vector<TopoRec> topo(m);
+    for (int64_t b = 0; b < n; ++b) node_body(b, sk.data(), n, ld.data(), sd.data(), off.data(), 2.0 / theta, topo.data());
+    for (int64_t i = 0; i < m; ++i)
+        if (topo_leaf(topo[i])) leaf_sort_body(topo[i], perm.data());
+    *m_out = m;
+    if (topo_out) std::copy(topo.begin(), topo.end(), topo_out);
+    if (perm_out) std::copy(perm.begin(), perm.end(), perm_out);
+    return m;
+}
+
+// keys of the sorted points are needed by the export; recompute them here
+extern "C" int64_t hl_export(const double* pts, int64_t n, const TopoRec* topo, int64_t m,
+                             const int32_t* perm, int64_t* parent, int64_t* children,
+                             uint8_t* is_leaf, double* half_w, int64_t* num_points,
+                             int64_t* n2p_index, int64_t* n2p_start) {
+    std::vector<uint64_t> sk(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const double* p = pts + 3 * perm[i];
+        sk[i] = morton_key(cell_index(p[0]), cell_index(p[1]), cell_index(p[2]));
+    }
+    return export_reference_layout(topo, m, perm, sk.data(), parent, children, is_leaf, half_w,
+                                   num_points, n2p_index, n2p_start);
+}
