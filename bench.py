@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""Benchmark of the treecode Gauss operator on one MI355X.
+
+Workload (the 1M-point configuration the headline metric is quoted on):
+1,000,000 points on a union of 6 seeded random ellipsoids (semi-axes
+U[0.2,0.6], centres U[-0.5,0.5]^3, noise sigma = 5e-4), normalised as
+CUDA-based/main_CAGR.py:42-45 does, float32; leaf size 32, theta = 0.5,
+smoothing width 0.002 for every point (the l0 preset's minimum).
+
+One step = one matvec pair: A.mu (A21, A22, A23 on mu [N,3], one fused
+upward pass + traversal) followed by A^T.r (A2T1, A2T2, A2T3 on the three
+columns of r = -A.mu, summed; one fused pass with three sets of centres), with
+the inputs resident in HBM.  Upward passes are inside the step; the tree
+build is outside (reported as build_ms).  The main figure is the
+FP32-compensated form; the plain FP64 operator on the same workload is timed
+beside it (fp64_value), and the oracle (the FP64 CPU restatement) is timed on
+a bounded sample of queries on the host cores (cpu_baseline).
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Gauss-operator matvecs/s (A+Aᵀ) at 1M pts; solve time vs plain FP64 at 1e-6"
+PEAK_FP32_VECTOR_TFLOPS = 157.3     # MI355X published FP32 vector peak
+F_OP = {"A2k": 25, "A2Tk": 27}      # algorithmic flops per interaction (sqrt, div, exp = 1)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=1_000_000)
+    ap.add_argument("--leaf", type=int, default=32)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--width", type=float, default=0.002)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU time of the oracle sample")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no ROCm GPU visible"}))
+        sys.exit(1)
+    from cagr_amd import WindingNumberTreecode, synth
+
+    dev = torch.device("cuda:0")
+    n = args.points
+    log(f"generating {n} points")
+    pts = synth.normalize(synth.ellipsoids(n, args.seed)).astype(np.float32)
+    mu_np = synth.random_normals(n, args.seed + 1).astype(np.float32)
+    pts_t = torch.from_numpy(pts).to(dev)
+    mu = torch.from_numpy(mu_np).to(dev)
+    w = torch.full((n,), args.width, dtype=torch.float32, device=dev)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    wt = WindingNumberTreecode(pts_t, leaf_size=args.leaf, theta=args.theta, precision="fp32c")
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    info = wt.info()
+    log(f"tree built in {build_ms:.1f} ms: {info}")
+
+    def step(prec):
+        a = wt.forward_A2(mu, w, precision=prec)
+        return wt.forward_A2T_sum(-a, w, precision=prec), a
+
+    def timed(prec, steps, warmup, profile=False):
+        for _ in range(warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        if profile:
+            wt.profile(True)
+        t = time.perf_counter()
+        for _ in range(steps):
+            out = step(prec)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        prof = None
+        if profile:
+            wt.profile(False)
+            prof = wt.profile_read()
+        return el, out, prof
+
+    # ---------------------------------------------------------------- FP32c
+    log(f"fp32c: {args.warmup} warmup + {args.steps} timed steps")
+    el, (r32, a32), prof = timed("fp32c", args.steps, args.warmup, profile=True)
+    value = args.steps / el
+    ms_per_step = el / args.steps * 1e3
+    trav = np.array(prof["traverse"]).reshape(args.steps, 2)   # [A2, A2T_sum] per step
+    upw = np.array(prof["upward"]).reshape(args.steps, 2)
+    t_a2, t_a2t = trav[:, 0].mean(), trav[:, 1].mean()
+    log(f"fp32c {value:.2f} pairs/s, {ms_per_step:.3f} ms/step; traversal A2 {t_a2:.3f} ms, "
+        f"A2T {t_a2t:.3f} ms; upward {upw[:, 0].mean():.3f} / {upw[:, 1].mean():.3f} ms")
+
+    # exact interaction counts of both traversals (same decisions in every precision)
+    far_a, near_a, _ = wt.counts("A2", mu)
+    s3 = (-a32).contiguous()
+    far_t, near_t, _ = wt.counts("A2T_sum", s3)
+    inter_a = int(far_a.sum().item() + near_a.sum().item())
+    inter_t = int(far_t.sum().item() + near_t.sum().item())
+    flop_a = inter_a * 3 * F_OP["A2k"]          # three directions per interaction
+    flop_t = inter_t * F_OP["A2Tk"]             # interactions already summed over the 3 sets
+    dom = "A2" if t_a2 >= t_a2t else "A2T_sum"
+    dom_flop, dom_ms = (flop_a, t_a2) if dom == "A2" else (flop_t, t_a2t)
+    achieved = dom_flop / (dom_ms * 1e-3) / 1e12
+    log(f"interactions per launch: A2 {inter_a:.3e}, A2T_sum {inter_t:.3e}; dominant {dom}: "
+        f"{achieved:.1f} TFLOP/s")
+
+    # ---------------------------------------------------------------- FP64
+    fp64 = None
+    if not args.no_fp64:
+        ks = args.fp64_steps or max(2, args.steps // 4)
+        log(f"fp64: 1 warmup + {ks} timed steps")
+        el64, (r64, a64), prof64 = timed("fp64", ks, 1, profile=True)
+        tr64 = np.array(prof64["traverse"]).reshape(ks, 2)
+        a_err = ((a32.double() - a64.double()).abs().max() / a64.double().abs().max()).item()
+        r_err = ((r32.double() - r64.double()).abs().max() / r64.double().abs().max()).item()
+        fp64 = dict(value=ks / el64, ms_per_step=el64 / ks * 1e3,
+                    traverse_ms=[float(tr64[:, 0].mean()), float(tr64[:, 1].mean())],
+                    fp32c_err_vs_fp64={"A_mu": a_err, "AT_r": r_err})
+        log(f"fp64 {fp64['value']:.2f} pairs/s; fp32c normwise err vs fp64: A {a_err:.2e}, AT {r_err:.2e}")
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pts, mu_np, s3.cpu().numpy().astype(np.float64), args, r32, a32)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "matvec pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "1M points on 6 seeded ellipsoids (sigma 5e-4), A.mu + A^T.r matvec pair, "
+                        "FP32 compensated, FP64 opening tests",
+            "points": n, "leaf_size": args.leaf, "theta": args.theta, "width": args.width,
+            "nodes": info["num_nodes"], "depth": info["depth"],
+        },
+        "build_ms": round(build_ms, 3),
+        "roofline": {
+            "bound": "valu",
+            "kernel": f"k_trav_{'v' if dom == 'A2' else 's'} ({dom}, fp32c)",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_VECTOR_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_VECTOR_TFLOPS, 4),
+            "traffic": None,
+            "flop_per_launch": dom_flop,
+            "launch_ms": round(float(dom_ms), 4),
+        },
+        "kernels_ms": {"traverse_A2": round(float(t_a2), 4), "traverse_A2T_sum": round(float(t_a2t), 4),
+                       "upward_A2": round(float(upw[:, 0].mean()), 4),
+                       "upward_A2T_sum": round(float(upw[:, 1].mean()), 4)},
+        "interactions": {"A2": inter_a, "A2T_sum": inter_t},
+    }
+    if fp64:
+        line["fp64_value"] = round(fp64["value"], 4)
+        line["speedup_vs_fp64"] = round(value / fp64["value"], 3)
+        line["fp64"] = fp64
+    if cpu:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(pts, mu_np, s3, args, r32, a32):
+    """The FP64 oracle (CPU restatement, OpenMP) on a bounded random sample of
+    queries of the same workload, extrapolated to all N queries."""
+    import oracle
+    n = pts.shape[0]
+    p64 = pts.astype(np.float64)
+    mu64 = mu_np.astype(np.float64)
+    L = oracle.lib()
+    cores = int(L.orc_num_threads())
+    t0 = time.perf_counter()
+    tree = oracle.build_tree(p64, 15, args.leaf)
+    t_build = time.perf_counter() - t0
+    log(f"cpu: oracle tree built in {t_build:.1f} s ({tree.num_nodes} nodes), {cores} threads")
+    # upward passes of one matvec pair (1 for A2k, 3 for A2Tk), full size
+    t0 = time.perf_counter()
+    mom_mu = oracle.upward(tree, p64, mu64)
+    mom_s = [oracle.upward(tree, p64, s3[:, k:k + 1]) for k in range(3)]
+    t_up = time.perf_counter() - t0
+
+    rng = np.random.default_rng(args.seed + 7)
+
+    def run(q_idx):
+        q = p64[q_idx]
+        wq = np.full(q.shape[0], args.width)
+        t = time.perf_counter()
+        outs = [oracle.apply(f"A2{k + 1}", tree, p64, mu64, wq, queries=q, node_moments=mom_mu,
+                             theta=args.theta) for k in range(3)]
+        outs_t = [oracle.apply(f"A2T{k + 1}", tree, p64, s3[:, k:k + 1], wq, queries=q,
+                               node_moments=mom_s[k], theta=args.theta) for k in range(3)]
+        return time.perf_counter() - t, np.concatenate(outs, 1), sum(outs_t)
+
+    q0 = rng.choice(n, 256, replace=False)
+    dt0, _, _ = run(q0)
+    per_q = dt0 / 256
+    m = int(min(n, max(256, args.cpu_seconds / max(per_q, 1e-9))))
+    q_idx = np.sort(rng.choice(n, m, replace=False))
+    dt, ref_a, ref_r = run(q_idx)
+    per_pair = t_up + dt * n / m
+    # accuracy of the GPU FP32c result on the sampled queries against the FP64 oracle
+    ga = a32.double().cpu().numpy()[q_idx]
+    gr = r32.double().cpu().numpy()[q_idx]
+    ea = float(np.abs(ga - ref_a).max() / np.abs(ref_a).max())
+    er = float(np.abs(gr - ref_r).max() / np.abs(ref_r).max())
+    log(f"cpu: {m} sampled queries in {dt:.1f} s + upward {t_up:.2f} s -> {1 / per_pair:.5f} pairs/s; "
+        f"gpu fp32c err vs oracle on sample: A {ea:.2e}, AT {er:.2e}")
+    return {
+        "value": round(1.0 / per_pair, 6),
+        "unit": "matvec pairs/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{m} random queries of the same 1M-point tree (all 6 operators), full-size "
+                  f"upward passes ({t_up:.2f} s), extrapolated linearly to {n} queries; "
+                  f"oracle tree build {t_build:.1f} s not included",
+        "gpu_fp32c_err_vs_oracle_on_sample": {"A_mu": ea, "AT_r": er},
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -30,7 +30,9 @@ OUT_DIM = {"A": 1, "A_cpu": 1, "AT": 3, "G": 3, "A21": 1, "A22": 1, "A23": 1, "A
 EXPORTS = (
     "cagr_last_error", "cagr_version", "cagr_tree_build", "cagr_tree_destroy", "cagr_tree_info",
     "cagr_tree_export", "cagr_tree_permutation", "cagr_upward", "cagr_apply",
-    "cagr_apply_weighted", "cagr_apply_queries", "cagr_grid_query", "cagr_count",
+    "cagr_apply_weighted", "cagr_apply_adjoint", "cagr_apply_queries", "cagr_grid_query",
+    "cagr_count",
+    "cagr_profile", "cagr_profile_read",
 )
 
 _P, _I, _D, _C = ct.c_void_p, ct.c_int64, ct.c_double, ct.c_int
@@ -45,9 +47,12 @@ _SIGS = {
     "cagr_upward": (_C, [_P, _P, _C, _C, _P, _P, _P, _P, _P]),
     "cagr_apply": (_C, [_P, _C, _C, _P, _C, _P, _C, _D, _P, _P]),
     "cagr_apply_weighted": (_C, [_P, _C, _C, _P, _C, _P, _P, _C, _D, _P, _P]),
+    "cagr_apply_adjoint": (_C, [_P, _C, _C, _P, _C, _P, _P, _C, _D, _P, _P]),
     "cagr_apply_queries": (_C, [_P, _C, _C, _P, _C, _P, _C, _I, _P, _D, _C, _D, _P, _P]),
     "cagr_grid_query": (_C, [_P, _C, _C, _P, _C, _P, _P, _P, _D, _C, _D, _P, _P]),
     "cagr_count": (_C, [_P, _C, _P, _C, _P, _P, _P, _P, _P]),
+    "cagr_profile": (_C, [_P, _C]),
+    "cagr_profile_read": (_C, [_P, _C, _P, _P, _P, _I]),
 }
 
 _lib = None

@@ -1,0 +1,415 @@
+// Exact adjoint of the linear treecode operator (gfx950).
+//
+// With node centres fixed by caller-given weights (cagr_apply_weighted), the
+// treecode y = A x of the vector-family operators (A / A_cpu, A2k, A3, fused
+// A2) is linear in x:
+//   y_i = sum_{S far for i} K(x_i - c_S) . X_S + sum_{j in near leaves of i} K(x_i - y_j) . x_j,
+//   X_S = sum_{j in S} x_j.
+// Its exact adjoint is
+//   (A^T g)_j = N_j + sum_{S containing j} L_S,
+//   L_S = sum_{i: S far for i} g_i K(x_i - c_S),   N_j = sum_{i: leaf(j) near for i} g_i K(x_i - y_j),
+// computed here with the forward traversal itself: the same wave-coherent
+// walk (same decisions, node for node), but every interaction is scattered
+// back to its source -- far nodes through a wave reduction and one FP64 atomic
+// per node visit into L, near leaves transposed (lanes hold the leaf's
+// sources, the wave's near queries are broadcast one by one) with one FP64
+// atomic per source and visit into N -- then a downward sweep adds the L of
+// every node on a point's root path.  Nothing here needs the reference's
+// A2Tk: that is the adjoint of the direction-reversed operator, not of A2k
+// (A2Tk(An) = (-A2k(-An))^T).
+#include "common.h"
+#include "trav_common.h"
+#include <climits>
+
+namespace cagr {
+namespace {
+
+using namespace dev;
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / 64;
+
+enum { GM_A = 0, GM_A2K = 1, GM_A3 = 2, GM_A2 = 3 };
+
+struct AP {
+    int64_t n;
+    int32_t root, stop;
+    const int32_t* perm;
+    const void* g;            // cotangent [n, gcols], original order
+    int gdtype;
+    int gmode;                // GM_*
+    int kdir;                 // GM_A2K: direction
+    const void* widths;
+    int wdtype;
+    double an, sign;
+    double* adj_node;         // [m,3] accumulators L
+    double* adj_pt;           // [n,3] accumulators N (sorted order)
+};
+
+// cotangent of query oi as the per-direction weights of the k-sum form
+// (GM_A: scalar in gk[0])
+__device__ inline void load_cot(const AP& p, int64_t oi, double gk[3]) {
+    if (p.gmode == GM_A2) {
+        for (int k = 0; k < 3; ++k) gk[k] = load_as_double(p.g, p.gdtype, 3 * oi + k);
+    } else {
+        const double v = load_as_double(p.g, p.gdtype, oi);
+        if (p.gmode == GM_A) {
+            gk[0] = v; gk[1] = 0.0; gk[2] = 0.0;
+        } else if (p.gmode == GM_A3) {
+            gk[0] = gk[1] = gk[2] = v / 3.0;
+        } else {
+            for (int k = 0; k < 3; ++k) gk[k] = (k == p.kdir) ? v : 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP32c
+// interaction of (query with factors h, width state) with a source at d:
+// returns the contribution to the source's adjoint accumulator
+template <bool GA, bool CONT>
+__device__ inline void adj_term32(float* z, const float* d, float r2, bool in, const float* h,
+                                  const float* e, float hanf, const float* gin, float winv,
+                                  float ew) {
+    const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+    const float ir3 = ir * ir * ir;
+    if constexpr (GA) {
+        const float u = h[0] * ir3;
+        for (int c = 0; c < 3; ++c) z[c] = fmaf(u, d[c], z[c]);
+        if constexpr (CONT) {
+            if (in) {
+                const float v = gin[0] * winv * winv * winv;
+                for (int c = 0; c < 3; ++c) z[c] = fmaf(v, d[c], z[c]);
+            }
+        }
+    } else {
+        const float a0 = h[0] * e[0], a1 = h[1] * e[1], a2 = h[2] * e[2];
+        const float gi3 = ((a0 + a1) + a2) * ir3;
+        const float hir = hanf * ir;
+        z[0] = fmaf(gi3, d[0], fmaf(a0, hir, z[0]));
+        z[1] = fmaf(gi3, d[1], fmaf(a1, hir, z[1]));
+        z[2] = fmaf(gi3, d[2], fmaf(a2, hir, z[2]));
+        if constexpr (CONT) {
+            if (in) {
+                const float gs = (gin[0] + gin[1]) + gin[2];
+                const float v = ew * gs * winv * winv * winv;
+                const float hw = ew * hanf * winv;
+                for (int c = 0; c < 3; ++c) z[c] = fmaf(v, d[c], fmaf(hw, gin[c], z[c]));
+            }
+        }
+    }
+}
+
+template <int GMODE_A, bool CONT>
+__global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restrict__ topo,
+                                                    const NodeV32* __restrict__ nodes,
+                                                    const SrcV32* __restrict__ src,
+                                                    const double* __restrict__ pts64,
+                                                    const float* __restrict__ pts32) {
+    constexpr bool GA = GMODE_A != 0;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t s = wave * 64 + lane;
+    const bool valid = s < p.n;
+    if (__ballot(valid) == 0) return;
+    const int64_t sc = valid ? s : 0;
+    const double x[3] = {pts64[3 * sc], pts64[3 * sc + 1], pts64[3 * sc + 2]};
+    const float xf[3] = {pts32[3 * sc], pts32[3 * sc + 1], pts32[3 * sc + 2]};
+    const int64_t oi = p.perm[sc];
+    const double w = load_as_double(p.widths, p.wdtype, oi);
+    double gk[3];
+    load_cot(p, oi, gk);
+    if (!valid) gk[0] = gk[1] = gk[2] = 0.0;
+    float h[3], gin[3];
+    if constexpr (GA) {
+        h[0] = (float)(p.sign * gk[0]);
+        h[1] = h[2] = 0.f;
+        gin[0] = h[0];
+        gin[1] = gin[2] = 0.f;
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            h[k] = (float)(gk[k] * exp(0.5 * p.an * x[k]));
+            gin[k] = (float)gk[k];
+        }
+    }
+    const float wf = (float)w, w2 = wf * wf, winv = 1.f / wf;
+    const float ew = CONT ? (float)exp(p.an * w) : 0.f;
+    const float hanf = (float)(0.5 * p.an);
+    int32_t resume = valid ? p.root : INT_MAX;
+
+    for (int32_t n = p.root; n < p.stop;) {
+        n = __builtin_amdgcn_readfirstlane(n);
+        const TopoRec t = topo[n];
+        const NodeV32 nr = nodes[n];
+        const bool act = resume <= n;
+        const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
+        const double d2 = dist2_rn(e0, e1, e2);
+        const bool far = d2 > t.thr2;
+        const bool takefar = act && far;
+        float z[3] = {0.f, 0.f, 0.f};
+        if (takefar) {
+            resume = t.escape;
+            const float d[3] = {(float)e0, (float)e1, (float)e2};
+            const float r2 = (float)d2;
+            const bool in = inside_fp32(r2, w2, e0, e1, e2, w);
+            adj_term32<GA, CONT>(z, d, r2, in, h, nr.e, hanf, gin, winv, ew);
+        }
+        if (__ballot(takefar) != 0) {
+            const float s0 = wave_sum(z[0]), s1 = wave_sum(z[1]), s2 = wave_sum(z[2]);
+            if (lane == 0) {
+                atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
+                atomicAdd(p.adj_node + 3 * (int64_t)n + 1, (double)s1);
+                atomicAdd(p.adj_node + 3 * (int64_t)n + 2, (double)s2);
+            }
+        }
+        const bool nearl = act && !far;
+        if (topo_leaf(t)) {
+            if (nearl) resume = t.escape;
+            const uint64_t qmask = __ballot(nearl);
+            if (qmask != 0) {
+                for (int32_t base = t.begin; base < t.end; base += 64) {
+                    const int32_t j = base + lane;
+                    const bool has = j < t.end;
+                    SrcV32 sr;
+                    if (has) sr = src[j];
+                    float zz[3] = {0.f, 0.f, 0.f};
+                    for (uint64_t mm = qmask; mm != 0; mm &= mm - 1) {
+                        const int qi = __builtin_ctzll(mm);
+                        const float qx[3] = {lane_bcast(xf[0], qi), lane_bcast(xf[1], qi),
+                                             lane_bcast(xf[2], qi)};
+                        const float qh[3] = {lane_bcast(h[0], qi), lane_bcast(h[1], qi),
+                                             lane_bcast(h[2], qi)};
+                        const float qw2 = lane_bcast(w2, qi);
+                        float qg[3] = {0.f, 0.f, 0.f}, qwinv = 0.f, qew = 0.f;
+                        if constexpr (CONT) {
+                            qg[0] = lane_bcast(gin[0], qi);
+                            qg[1] = lane_bcast(gin[1], qi);
+                            qg[2] = lane_bcast(gin[2], qi);
+                            qwinv = lane_bcast(winv, qi);
+                            qew = lane_bcast(ew, qi);
+                        }
+                        if (has) {
+                            const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
+                            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                            const bool amb = fabsf(r2 - qw2) <= 1e-5f * qw2;
+                            bool in = r2 < qw2;
+                            if (__builtin_expect(__ballot(amb) != 0, 0)) {
+                                const double qx64[3] = {lane_bcast(x[0], qi), lane_bcast(x[1], qi),
+                                                        lane_bcast(x[2], qi)};
+                                const double qw64 = lane_bcast(w, qi);
+                                if (amb)
+                                    in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
+                                                         qx64[1] - (double)sr.y[1],
+                                                         qx64[2] - (double)sr.y[2])) >= qw64);
+                            }
+                            adj_term32<GA, CONT>(zz, d, r2, in, qh, sr.e, hanf, qg, qwinv, qew);
+                        }
+                    }
+                    if (has) {
+                        atomicAdd(p.adj_pt + 3 * (int64_t)j, (double)zz[0]);
+                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 1, (double)zz[1]);
+                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 2, (double)zz[2]);
+                    }
+                }
+            }
+            n = t.escape;
+        } else {
+            n = __ballot(nearl) != 0 ? n + 1 : t.escape;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP64
+template <bool GA, bool CONT>
+__device__ inline void adj_term64(double* z, const double* d, double w, const double* gk,
+                                  double an, double sign) {
+    const double r2 = dist2_rn(d[0], d[1], d[2]);
+    const double r = sqrt(r2);
+    if (r >= w) {
+        const double ir = 1.0 / r;
+        const double ir3 = ir * ir * ir;
+        if constexpr (GA) {
+            const double u = sign * gk[0] * ir3;
+            for (int c = 0; c < 3; ++c) z[c] += u * d[c];
+        } else {
+            const double a0 = gk[0] * exp(0.5 * an * d[0]);
+            const double a1 = gk[1] * exp(0.5 * an * d[1]);
+            const double a2 = gk[2] * exp(0.5 * an * d[2]);
+            const double gi3 = ((a0 + a1) + a2) * ir3;
+            const double hir = 0.5 * an * ir;
+            z[0] += gi3 * d[0] + a0 * hir;
+            z[1] += gi3 * d[1] + a1 * hir;
+            z[2] += gi3 * d[2] + a2 * hir;
+        }
+    } else if constexpr (CONT) {
+        const double w3 = w * w * w;
+        if constexpr (GA) {
+            const double u = sign * gk[0] / w3;
+            for (int c = 0; c < 3; ++c) z[c] += u * d[c];
+        } else {
+            const double ew = exp(an * w);
+            const double gs = (gk[0] + gk[1]) + gk[2];
+            for (int c = 0; c < 3; ++c) z[c] += ew * (gs * d[c] / w3 + 0.5 * an * gk[c] / w);
+        }
+    }
+}
+
+template <int GMODE_A, bool CONT>
+__global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restrict__ topo,
+                                                    const NodeV64* __restrict__ nodes,
+                                                    const SrcV64* __restrict__ src,
+                                                    const double* __restrict__ pts64) {
+    constexpr bool GA = GMODE_A != 0;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t s = wave * 64 + lane;
+    const bool valid = s < p.n;
+    if (__ballot(valid) == 0) return;
+    const int64_t sc = valid ? s : 0;
+    const double x[3] = {pts64[3 * sc], pts64[3 * sc + 1], pts64[3 * sc + 2]};
+    const int64_t oi = p.perm[sc];
+    const double w = load_as_double(p.widths, p.wdtype, oi);
+    double gk[3];
+    load_cot(p, oi, gk);
+    if (!valid) gk[0] = gk[1] = gk[2] = 0.0;
+    int32_t resume = valid ? p.root : INT_MAX;
+
+    for (int32_t n = p.root; n < p.stop;) {
+        n = __builtin_amdgcn_readfirstlane(n);
+        const TopoRec t = topo[n];
+        const NodeV64 nr = nodes[n];
+        const bool act = resume <= n;
+        const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
+        const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
+        const bool takefar = act && far;
+        double z[3] = {0.0, 0.0, 0.0};
+        if (takefar) {
+            resume = t.escape;
+            adj_term64<GA, CONT>(z, d, w, gk, p.an, p.sign);
+        }
+        if (__ballot(takefar) != 0) {
+            const double s0 = wave_sum(z[0]), s1 = wave_sum(z[1]), s2 = wave_sum(z[2]);
+            if (lane == 0) {
+                atomicAdd(p.adj_node + 3 * (int64_t)n, s0);
+                atomicAdd(p.adj_node + 3 * (int64_t)n + 1, s1);
+                atomicAdd(p.adj_node + 3 * (int64_t)n + 2, s2);
+            }
+        }
+        const bool nearl = act && !far;
+        if (topo_leaf(t)) {
+            if (nearl) resume = t.escape;
+            const uint64_t qmask = __ballot(nearl);
+            if (qmask != 0) {
+                for (int32_t base = t.begin; base < t.end; base += 64) {
+                    const int32_t j = base + lane;
+                    const bool has = j < t.end;
+                    SrcV64 sr;
+                    if (has) sr = src[j];
+                    double zz[3] = {0.0, 0.0, 0.0};
+                    for (uint64_t mm = qmask; mm != 0; mm &= mm - 1) {
+                        const int qi = __builtin_ctzll(mm);
+                        const double qx[3] = {lane_bcast(x[0], qi), lane_bcast(x[1], qi),
+                                              lane_bcast(x[2], qi)};
+                        const double qg[3] = {lane_bcast(gk[0], qi), lane_bcast(gk[1], qi),
+                                              lane_bcast(gk[2], qi)};
+                        const double qw = lane_bcast(w, qi);
+                        if (has) {
+                            const double dd[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
+                            adj_term64<GA, CONT>(zz, dd, qw, qg, p.an, p.sign);
+                        }
+                    }
+                    if (has) {
+                        atomicAdd(p.adj_pt + 3 * (int64_t)j, zz[0]);
+                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 1, zz[1]);
+                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 2, zz[2]);
+                    }
+                }
+            }
+            n = t.escape;
+        } else {
+            n = __ballot(nearl) != 0 ? n + 1 : t.escape;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill0(double* __restrict__ a, int64_t len) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < len) a[i] = 0.0;
+}
+
+// out[perm[s]] = N_s + sum of L over the nodes containing s
+template <typename TO>
+__global__ __launch_bounds__(kThreads) void k_adj_down(int64_t n, const int32_t* __restrict__ perm,
+                                                       const int32_t* __restrict__ leaf_of,
+                                                       const TopoRec* __restrict__ topo,
+                                                       const double* __restrict__ adj_node,
+                                                       const double* __restrict__ adj_pt,
+                                                       TO* __restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (s >= n) return;
+    double z[3] = {adj_pt[3 * s], adj_pt[3 * s + 1], adj_pt[3 * s + 2]};
+    for (int32_t nd = leaf_of[s]; nd >= 0; nd = topo[nd].parent)
+        for (int c = 0; c < 3; ++c) z[c] += adj_node[3 * (int64_t)nd + c];
+    const int64_t o = perm[s];
+    for (int c = 0; c < 3; ++c) out[3 * o + c] = (TO)z[c];
+}
+
+}  // namespace
+
+int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
+            int continuous, double an, void* out, hipStream_t st) {
+    AP p{};
+    p.n = t->n;
+    p.root = 0;
+    p.stop = (int32_t)t->m;
+    p.perm = t->perm;
+    p.g = g;
+    p.gdtype = dtype;
+    p.widths = widths;
+    p.wdtype = dtype;
+    p.an = an;
+    p.sign = op == CAGR_OP_A_CPU ? -1.0 : 1.0;
+    p.adj_node = t->adj_node;
+    p.adj_pt = t->adj_pt;
+    p.kdir = 0;
+    if (op == CAGR_OP_A || op == CAGR_OP_A_CPU) p.gmode = GM_A;
+    else if (op == CAGR_OP_A3) p.gmode = GM_A3;
+    else if (op == CAGR_OP_A2) p.gmode = GM_A2;
+    else { p.gmode = GM_A2K; p.kdir = op - CAGR_OP_A21; }
+    const bool ga = p.gmode == GM_A;
+    const int64_t n = t->n, m = t->m;
+    k_fill0<<<(3 * m + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_node, 3 * m);
+    CAGR_LAUNCHED();
+    k_fill0<<<(3 * n + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_pt, 3 * n);
+    CAGR_LAUNCHED();
+    const int64_t blocks = ((n + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (precision == CAGR_PREC_FP32C) {
+        auto nodes = (const NodeV32*)t->nodes;
+        auto src = (const SrcV32*)t->src;
+        if (ga) {
+            if (continuous) k_adj32<1, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+            else k_adj32<1, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+        } else {
+            if (continuous) k_adj32<0, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+            else k_adj32<0, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+        }
+    } else {
+        auto nodes = (const NodeV64*)t->nodes;
+        auto src = (const SrcV64*)t->src;
+        if (ga) {
+            if (continuous) k_adj64<1, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+            else k_adj64<1, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+        } else {
+            if (continuous) k_adj64<0, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+            else k_adj64<0, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+        }
+    }
+    CAGR_LAUNCHED();
+    const int64_t nb = (n + kThreads - 1) / kThreads;
+    if (dtype == CAGR_F32)
+        k_adj_down<float><<<nb, kThreads, 0, st>>>(n, t->perm, t->leaf_of, t->topo, t->adj_node, t->adj_pt, (float*)out);
+    else
+        k_adj_down<double><<<nb, kThreads, 0, st>>>(n, t->perm, t->leaf_of, t->topo, t->adj_node, t->adj_pt, (double*)out);
+    CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+
+}  // namespace cagr

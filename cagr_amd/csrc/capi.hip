@@ -10,6 +10,19 @@ namespace cagr {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
+int prof_mark(cagr_tree* t, int kind, hipStream_t st) {
+    if (!t->profiling) return CAGR_OK;
+    auto& ev = t->prof_events[kind];
+    size_t& used = t->prof_used[kind];
+    if (used == ev.size()) {
+        hipEvent_t e;
+        CAGR_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+    }
+    CAGR_HIP(hipEventRecord(ev[used++], st));
+    return CAGR_OK;
+}
+
 // Morton order of arbitrary query points (implemented in tree_build.hip)
 int morton_order(const void* pts, int dtype, int64_t n, int32_t* order, void* scratch,
                  hipStream_t st);
@@ -69,14 +82,19 @@ int apply_impl(cagr_tree* t, int op, int precision, const void* attrs, int dtype
     if (t->n == 0) return CAGR_OK;
     CAGR_REQUIRE(attrs && widths && out, CAGR_ERR_ARG, "null attrs / widths / out");
     hipStream_t st = (hipStream_t)stream;
+    if ((rc = prof_mark(t, 1, st))) return rc;
     rc = run_upward_for(t, op, precision, oi, attrs, dtype, weights, an, st);
     if (rc) return rc;
+    if ((rc = prof_mark(t, 1, st))) return rc;
     Queries q;
     q.mode = 0;
     q.nq = t->n;
     q.widths = widths;
     q.wdtype = dtype;
-    return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
+    if ((rc = prof_mark(t, 0, st))) return rc;
+    rc = traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
+    if (rc) return rc;
+    return prof_mark(t, 0, st);
 }
 
 }  // namespace
@@ -116,8 +134,35 @@ int cagr_tree_build(const void* points, int64_t n, int dtype, int max_depth, int
     return CAGR_OK;
 }
 
+int cagr_profile(cagr_tree* t, int enable) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    t->profiling = enable != 0;
+    if (enable) t->prof_used[0] = t->prof_used[1] = 0;
+    return CAGR_OK;
+}
+
+int cagr_profile_read(cagr_tree* t, int kind, int64_t* launches, double* total_ms,
+                      double* each_ms, int64_t cap) {
+    CAGR_REQUIRE(t != nullptr && (kind == 0 || kind == 1), CAGR_ERR_ARG, "bad argument");
+    const auto& ev = t->prof_events[kind];
+    const size_t used = t->prof_used[kind];
+    double ms = 0.0;
+    for (size_t i = 0; i + 1 < used; i += 2) {
+        CAGR_HIP(hipEventSynchronize(ev[i + 1]));
+        float e = 0.f;
+        CAGR_HIP(hipEventElapsedTime(&e, ev[i], ev[i + 1]));
+        ms += e;
+        if (each_ms && (int64_t)(i / 2) < cap) each_ms[i / 2] = e;
+    }
+    if (launches) *launches = (int64_t)(used / 2);
+    if (total_ms) *total_ms = ms;
+    return CAGR_OK;
+}
+
 int cagr_tree_destroy(cagr_tree* t) {
     if (!t) return CAGR_OK;
+    for (auto& ev : t->prof_events)
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     (void)hipFree(t->pts64);
     (void)hipFree(t->pts32);
     (void)hipFree(t->perm);
@@ -130,6 +175,9 @@ int cagr_tree_destroy(cagr_tree* t) {
     (void)hipFree(t->r64);
     (void)hipFree(t->m64);
     (void)hipFree(t->qws);
+    (void)hipFree(t->leaf_of);
+    (void)hipFree(t->adj_node);
+    (void)hipFree(t->adj_pt);
     delete t;
     return CAGR_OK;
 }
@@ -210,6 +258,29 @@ int cagr_apply_weighted(cagr_tree* t, int op, int precision, const void* attrs, 
                         const void* weights, const void* widths, int continuous, double an,
                         void* out, void* stream) {
     return apply_impl(t, op, precision, attrs, dtype, weights, widths, continuous, an, out, stream);
+}
+
+int cagr_apply_adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype,
+                       const void* weights, const void* widths, int continuous, double an,
+                       void* out, void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, precision, dtype, &oi);
+    if (rc) return rc;
+    CAGR_REQUIRE(op == CAGR_OP_A || op == CAGR_OP_A_CPU || op == CAGR_OP_A3 || op == CAGR_OP_A2 ||
+                     (op >= CAGR_OP_A21 && op <= CAGR_OP_A23),
+                 CAGR_ERR_ARG, "adjoint supports A, A_CPU, A21..A23, A2 and A3");
+    if (t->n == 0) return CAGR_OK;
+    CAGR_REQUIRE(g && weights && widths && out, CAGR_ERR_ARG, "null g / weights / widths / out");
+    hipStream_t st = (hipStream_t)stream;
+    // node centres from the fixed weights (the moments are not needed)
+    if ((rc = prof_mark(t, 1, st))) return rc;
+    rc = upward(t, 0, precision, 1, 0, nullptr, dtype, weights, oi.convects ? an : 0.0, st);
+    if (rc) return rc;
+    if ((rc = prof_mark(t, 1, st))) return rc;
+    if ((rc = prof_mark(t, 0, st))) return rc;
+    rc = adjoint(t, op, precision, g, dtype, widths, continuous, an, out, st);
+    if (rc) return rc;
+    return prof_mark(t, 0, st);
 }
 
 int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, int dtype,

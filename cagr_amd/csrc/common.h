@@ -87,9 +87,22 @@ struct cagr_tree {
     double* w64 = nullptr;        // [3, m]
     double* r64 = nullptr;        // [3, m, 3]
     double* m64 = nullptr;        // [3, m, 3]
+    int32_t* leaf_of = nullptr;   // [n] leaf id of each sorted point
+    double* adj_node = nullptr;   // [m, 3] adjoint far-field accumulators
+    double* adj_pt = nullptr;     // [n, 3] adjoint near-field accumulators
     size_t ws_queries = 0;        // grow-only query workspace (explicit-query sorting)
     void* qws = nullptr;
+
+    // opt-in launch timing (cagr_profile): event pairs per kind, reused
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_events[2];
+    size_t prof_used[2] = {0, 0};
 };
+
+namespace cagr {
+// record an event of the current region (begin/end alternate) when profiling
+int prof_mark(cagr_tree* t, int kind, hipStream_t st);
+}
 
 namespace cagr {
 // implemented in tree_build.hip
@@ -114,6 +127,10 @@ struct Queries {
     int wdtype = CAGR_F32;
     double width = 0.0;           // used when widths == null
 };
+// implemented in adjoint.hip (needs the node records of an upward pass with
+// the same fixed weights)
+int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
+            int continuous, double an, void* out, hipStream_t st);
 int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuous, double an,
              void* out, int odtype, int64_t* cnt_far, int64_t* cnt_near, int64_t* cnt_visit,
              hipStream_t st);

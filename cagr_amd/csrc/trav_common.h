@@ -1,0 +1,74 @@
+// Device helpers shared by the traversal kernels (traverse.hip, adjoint.hip).
+#pragma once
+#include "common.h"
+
+namespace cagr {
+namespace dev {
+
+// workgroup -> logical block so that each XCD (workgroups are dealt to the 8
+// XCDs round-robin) gets one contiguous, spatially coherent range of blocks
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
+__device__ inline double load_as_double(const void* p, int dtype, int64_t i) {
+    return dtype == CAGR_F32 ? (double)((const float*)p)[i] : ((const double*)p)[i];
+}
+
+// squared distance, each operation rounded (the reference's order:
+// d0^2 + d1^2 + d2^2 from the left)
+__device__ inline double dist2_rn(double d0, double d1, double d2) {
+#pragma clang fp contract(off)
+    return (d0 * d0 + d1 * d1) + d2 * d2;
+}
+
+// a[i] for a small runtime i without indexing a register array (which would
+// spill the array to scratch)
+template <typename T>
+__device__ inline T sel3(const T* a, int i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+
+__device__ inline void neumaier(float& s, float& c, float v) {
+    const float t = s + v;
+    c += (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
+    s = t;
+}
+
+// r >= w decided as the FP64 reference would: FP32 unless too close to call
+__device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, double xd2,
+                                   double w) {
+    const bool amb = fabsf(r2 - w2) <= 1e-5f * w2;
+    bool in = r2 < w2;
+    if (__builtin_expect(__ballot(amb) != 0, 0)) {
+        if (amb) in = !(sqrt(dist2_rn(xd0, xd1, xd2)) >= w);
+    }
+    return in;
+}
+
+// sum of v over the 64 lanes of the wave (every lane gets the total); all
+// lanes must be active
+__device__ inline float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T>
+__device__ inline T lane_bcast(T v, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+    } else {
+        const int64_t b = __builtin_bit_cast(int64_t, v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+        return __builtin_bit_cast(T, ((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+    }
+}
+
+}  // namespace dev
+}  // namespace cagr

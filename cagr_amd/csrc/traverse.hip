@@ -33,6 +33,7 @@
 // records), so the FP32 form needs no exp per interaction.  The FP64 form
 // evaluates exp per interaction like the reference.
 #include "common.h"
+#include "trav_common.h"
 #include <climits>
 
 namespace cagr {
@@ -66,27 +67,11 @@ struct TP {
     int64_t* cnt_visit;
 };
 
-// workgroup -> logical block so that each XCD (workgroups are dealt to the 8
-// XCDs round-robin) gets one contiguous, spatially coherent range of blocks
-__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
-    const int64_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
-    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
-
-__device__ inline double load_as_double(const void* p, int dtype, int64_t i) {
-    return dtype == CAGR_F32 ? (double)((const float*)p)[i] : ((const double*)p)[i];
-}
+using namespace dev;
 
 __device__ inline void store_out(const TP& p, int64_t i, double v) {
     if (p.odtype == CAGR_F32) ((float*)p.out)[i] = (float)v;
     else ((double*)p.out)[i] = v;
-}
-
-// squared distance, each operation rounded (the reference's order:
-// d0^2 + d1^2 + d2^2 from the left)
-__device__ inline double dist2_rn(double d0, double d1, double d2) {
-#pragma clang fp contract(off)
-    return (d0 * d0 + d1 * d1) + d2 * d2;
 }
 
 template <int QM>
@@ -122,30 +107,6 @@ __device__ inline bool load_query(const TP& p, const double* __restrict__ pts64,
         w = p.width;
     }
     return valid;
-}
-
-// a[i] for a small runtime i without indexing a register array (which would
-// spill the array to scratch)
-template <typename T>
-__device__ inline T sel3(const T* a, int i) {
-    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
-}
-
-__device__ inline void neumaier(float& s, float& c, float v) {
-    const float t = s + v;
-    c += (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
-    s = t;
-}
-
-// r >= w decided as the FP64 reference would: FP32 unless too close to call
-__device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, double xd2,
-                                   double w) {
-    const bool amb = fabsf(r2 - w2) <= 1e-5f * w2;
-    bool in = r2 < w2;
-    if (__builtin_expect(__ballot(amb) != 0, 0)) {
-        if (amb) in = !(sqrt(dist2_rn(xd0, xd1, xd2)) >= w);
-    }
-    return in;
 }
 
 // ============================================================================

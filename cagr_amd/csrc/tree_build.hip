@@ -172,6 +172,15 @@ __global__ __launch_bounds__(kThreads) void k_gather_points(const T* __restrict_
     }
 }
 
+__global__ __launch_bounds__(kThreads) void k_leaf_of(const TopoRec* __restrict__ topo, int64_t m,
+                                                      int32_t* __restrict__ leaf_of) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= m) return;
+    const TopoRec t = topo[i];
+    if (topo_leaf(t))
+        for (int32_t s = t.begin; s < t.end; ++s) leaf_of[s] = (int32_t)i;
+}
+
 // group of a node in the upward order: 0 = leaf, 1 + (14 - depth) = internal
 __device__ inline int up_group(const TopoRec& t) {
     return topo_leaf(t) ? 0 : 1 + (kLevels - 1 - topo_depth(t));
@@ -348,6 +357,11 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     CAGR_HIP(hipMalloc(&t->w64, 3 * m * sizeof(double)));
     CAGR_HIP(hipMalloc(&t->r64, 9 * m * sizeof(double)));
     CAGR_HIP(hipMalloc(&t->m64, 9 * m * sizeof(double)));
+    CAGR_HIP(hipMalloc(&t->leaf_of, n * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&t->adj_node, 3 * m * sizeof(double)));
+    CAGR_HIP(hipMalloc(&t->adj_pt, 3 * n * sizeof(double)));
+    k_leaf_of<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->leaf_of);
+    CAGR_LAUNCHED();
     CAGR_HIP(hipStreamSynchronize(st));
     return CAGR_OK;
 }

@@ -25,8 +25,9 @@ namespace {
 constexpr int kThreads = 256;
 inline int64_t blocks_for(int64_t n) { return (n + kThreads - 1) / kThreads; }
 
+// attrs may be null (centres only, from fixed weights): read as zero
 template <typename TI>
-__device__ inline double ld64(const TI* p, int64_t i) { return (double)p[i]; }
+__device__ inline double ld64(const TI* p, int64_t i) { return p ? (double)p[i] : 0.0; }
 
 // record writers --------------------------------------------------------------
 template <int FAM, int K, int P>

@@ -1,0 +1,179 @@
+"""CAGR orientation solves on the GPU treecode.
+
+* ``cagr_schedule`` -- the reference's own 40-iteration solve
+  (CUDA-based/main_CAGR.py:90-174: 31 steepest-descent steps on the normal
+  equations with the reference "transpose" A2Tk, a residual reset, 9 CG steps,
+  then 8 WNNC G-steps with rescaling), quirks included (the [N,1] sum in A_r,
+  the width that keeps shrinking in the CG phase).  A21/A22/A23 run as one
+  fused operator and the three A2Tk as one fused sum -- the same decisions and
+  moments as the separate calls.
+* ``cgnr_solve`` -- the tolerance solve of the benchmark: CG on the normal
+  equations A^T A mu = A^T b of the linear operator A mu = -[A21 mu; A22 mu;
+  A23 mu] (node centres fixed, unit point weights, one width), with the exact
+  adjoint kernel for A^T, FP64 vectors and scalars, stopping when
+  ||A^T(b - A mu)|| / ||A^T b|| <= tol in FP64.
+* ``cgnr_solve_mixed`` -- the same solve with the FP32-compensated operator
+  doing the iterations: FP64 iterative refinement whose outer residuals
+  (and the stopping test) use the FP64 operator, and whose inner corrections
+  run CGNR with the FP32c operator; all CG vectors, dot products and updates
+  stay FP64.
+"""
+from __future__ import annotations
+
+import time
+from typing import Optional
+
+import torch
+import torch.nn.functional as F
+
+from .wn_treecode import WindingNumberTreecode
+
+PRESETS = {"l0": (0.002, 0.016), "l1": (0.01, 0.04), "l2": (0.02, 0.08), "l3": (0.03, 0.12),
+           "l4": (0.04, 0.16), "l5": (0.05, 0.2)}
+
+
+def cagr_schedule(wt: WindingNumberTreecode, wsmin: float, wsmax: float, iters: int = 40,
+                  n_sd: int = 30, wnnc_steps: int = 8, an: float = 1.0,
+                  precision: Optional[str] = None, dtype: Optional[torch.dtype] = None):
+    """main_CAGR.py:90-174.  Returns (normals [N,3], {'r': ||r||, 'rk1': ||rk1||})."""
+    dev = wt.device
+    dtype = dtype or wt.points.dtype
+    n = wt.points.shape[0]
+    widths = torch.ones(n, dtype=dtype, device=dev)
+    b3 = torch.full((n, 3), 0.5, dtype=dtype, device=dev)
+
+    def A2(mu, ws):
+        return wt.apply("A2", mu, widths * ws, an=an, precision=precision)
+
+    def A2T(s3, ws):
+        return wt.apply("A2T_sum", s3, widths * ws, an=an, precision=precision)
+
+    normals = torch.zeros((n, 3), dtype=dtype, device=dev)
+    ws = wsmin + 0.5 * (wsmax - wsmin)
+    rk1 = A2T(b3, ws)     # computed and overwritten before use, as the reference does
+    pk = rk1
+    r = rk1
+    for i in range(iters):
+        ws = wsmin + ((iters - 1 - i) / (iters - 1)) * (wsmax - wsmin)
+        if i <= n_sd:
+            a_mu = -A2(normals, ws)
+            r = A2T(b3, ws) - A2T(a_mu, ws) - 0.0 * normals
+            a_r = -(A2(r, ws).sum(1, keepdim=True))
+            alpha = (r * r).sum() / (a_r * a_r).sum()
+            normals = normals + alpha * r
+            if i == n_sd:
+                a_mu = -A2(normals, ws)
+                rk1 = A2T(b3, ws) - A2T(a_mu, ws) - 0.0 * normals
+                pk = rk1
+        else:
+            at_a_pk = A2T(-A2(pk, ws), ws)
+            alpha = (rk1 * rk1).sum() / (at_a_pk * pk).sum()
+            normals = normals + alpha * pk
+            rk20 = rk1 - alpha * at_a_pk
+            beta = (rk20 * rk20).sum() / (rk1 * rk1).sum()
+            pk = rk20 + beta * pk
+            rk1 = rk20
+    res = {"r": float(torch.linalg.norm(r)), "rk1": float(torch.linalg.norm(rk1))}
+    for _ in range(wnnc_steps):
+        out = wt.apply("G", normals, widths * ws, precision=precision)
+        out = F.normalize(out, dim=-1).contiguous()
+        normals = out * torch.linalg.norm(normals, dim=-1, keepdim=True)
+    return normals, res
+
+
+class NormalOperator:
+    """A mu = -[A21 mu; A22 mu; A23 mu] (as [N,3]) with node centres fixed by
+    unit weights, and its exact adjoint; counts operator applications."""
+
+    def __init__(self, wt: WindingNumberTreecode, width: float, an: float = 1.0):
+        n = wt.points.shape[0]
+        self.wt, self.an = wt, an
+        self.w64 = torch.full((n,), width, dtype=torch.float64, device=wt.device)
+        self.ones64 = torch.ones(n, dtype=torch.float64, device=wt.device)
+        self.calls = {"fp64": 0, "fp32c": 0}
+
+    def A(self, mu, precision):
+        self.calls[precision] += 1
+        return -self.wt.apply("A2", mu, self.w64, weights=self.ones64, an=self.an,
+                              precision=precision)
+
+    def AT(self, g, precision):
+        self.calls[precision] += 1
+        return -self.wt.apply_adjoint("A2", g, self.w64, self.ones64, an=self.an,
+                                      precision=precision)
+
+    def normal(self, p, precision):
+        return self.AT(self.A(p, precision), precision)
+
+
+def _dot(a, b):
+    return float((a * b).sum())
+
+
+def _cg(op: NormalOperator, rhs, x0, tol_abs, max_iter, precision):
+    """CG on A^T A x = rhs (FP64 vectors); stops on the recurrence residual."""
+    x = x0.clone()
+    r = rhs - op.normal(x, precision) if x0.abs().max() > 0 else rhs.clone()
+    p = r.clone()
+    rr = _dot(r, r)
+    it = 0
+    while it < max_iter and rr ** 0.5 > tol_abs:
+        q = op.normal(p, precision)
+        alpha = rr / _dot(p, q)
+        x += alpha * p
+        r -= alpha * q
+        rr_new = _dot(r, r)
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+        it += 1
+    return x, it
+
+
+def cgnr_solve(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
+               max_iter: int = 5000, an: float = 1.0):
+    """Plain FP64 CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol."""
+    op = NormalOperator(wt, width, an)
+    n = wt.points.shape[0]
+    bt = torch.full((n, 3), b, dtype=torch.float64, device=wt.device)
+    t0 = time.perf_counter()
+    rhs = op.AT(bt, "fp64")
+    nrhs = float(torch.linalg.norm(rhs))
+    mu, iters = _cg(op, rhs, torch.zeros_like(rhs), tol * nrhs, max_iter, "fp64")
+    true_res = float(torch.linalg.norm(op.AT(bt - op.A(mu, "fp64"), "fp64"))) / nrhs
+    torch.cuda.synchronize()
+    return mu, {"iters": iters, "rel_residual": true_res, "seconds": time.perf_counter() - t0,
+                "calls": dict(op.calls)}
+
+
+def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
+                     inner_tol: float = 1e-3, max_outer: int = 20, max_inner: int = 5000,
+                     an: float = 1.0):
+    """FP64 iterative refinement around FP32c CGNR corrections; the stopping
+    test uses the FP64 operator and exact adjoint."""
+    op = NormalOperator(wt, width, an)
+    n = wt.points.shape[0]
+    bt = torch.full((n, 3), b, dtype=torch.float64, device=wt.device)
+    t0 = time.perf_counter()
+    rhs = op.AT(bt, "fp64")
+    nrhs = float(torch.linalg.norm(rhs))
+    mu = torch.zeros_like(rhs)
+    r = rhs.clone()
+    res = 1.0
+    inner_total, outer = 0, 0
+    history = []
+    while outer < max_outer:
+        res = float(torch.linalg.norm(r)) / nrhs
+        history.append(res)
+        if res <= tol:
+            break
+        # correction: A^T A d = r (FP32c operator), to inner_tol relative
+        goal = max(inner_tol, 0.5 * tol / res)
+        d, it = _cg(op, r, torch.zeros_like(r), goal * float(torch.linalg.norm(r)), max_inner, "fp32c")
+        inner_total += it
+        mu += d
+        r = op.AT(bt - op.A(mu, "fp64"), "fp64")
+        outer += 1
+    torch.cuda.synchronize()
+    return mu, {"outer": outer, "inner_iters": inner_total, "rel_residual": history[-1] if res <= tol
+                else float(torch.linalg.norm(r)) / nrhs, "history": history,
+                "seconds": time.perf_counter() - t0, "calls": dict(op.calls)}

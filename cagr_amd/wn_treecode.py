@@ -117,6 +117,23 @@ class WindingNumberTreecode:
     node2point_index = property(lambda self: self._layout_tensor(5))
     node2point_indexstart = property(lambda self: self._layout_tensor(6))
 
+    def profile(self, enable: bool) -> None:
+        """Start (clear) or stop HIP-event timing of this tree's launches."""
+        check(self.L.cagr_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        """{'traverse': [ms per launch], 'upward': [ms per upward pass]} in launch
+        order since profile(True)."""
+        res = {}
+        for kind, name in ((0, "traverse"), (1, "upward")):
+            n, ms = ct.c_int64(), ct.c_double()
+            check(self.L.cagr_profile_read(self._h, kind, ct.byref(n), ct.byref(ms), None, 0))
+            each = np.zeros(max(n.value, 1))
+            check(self.L.cagr_profile_read(self._h, kind, ct.byref(n), ct.byref(ms),
+                                           each.ctypes.data_as(ct.c_void_p), n.value))
+            res[name] = each[: n.value].tolist()
+        return res
+
     def permutation(self) -> np.ndarray:
         """Sorted position -> original index (the leaf-preorder point order)."""
         perm = np.empty(self.points.shape[0], np.int32)
@@ -154,6 +171,24 @@ class WindingNumberTreecode:
                                                 _ptr(widths), int(continuous), float(an),
                                                 _ptr(out), _stream(self.device))
         check(rc)
+        return out
+
+    def apply_adjoint(self, op: str, g: torch.Tensor, widths: torch.Tensor, weights: torch.Tensor,
+                      *, continuous=False, an: float = 1.0,
+                      precision: Optional[str] = None) -> torch.Tensor:
+        """Exact adjoint of ``apply(op, ., widths, weights=weights)`` (node centres
+        fixed by ``weights``): g [N, out_dim] -> [N, 3]."""
+        n = self.points.shape[0]
+        g = g.contiguous()
+        assert g.shape == (n, OUT_DIM[op]), (op, tuple(g.shape))
+        widths = widths.contiguous().to(g.dtype)
+        weights = weights.contiguous().to(g.dtype)
+        assert widths.shape == (n,) and weights.shape == (n,)
+        out = torch.empty((n, 3), dtype=g.dtype, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.L.cagr_apply_adjoint(self._h, OP[op], self._prec(precision), _ptr(g), _dt(g),
+                                            _ptr(weights), _ptr(widths), int(continuous), float(an),
+                                            _ptr(out), _stream(self.device)))
         return out
 
     def upward(self, attrs: torch.Tensor, weights: Optional[torch.Tensor] = None):

@@ -117,6 +117,17 @@ int cagr_apply_weighted(cagr_tree* tree, int op, int precision, const void* attr
                         const void* weights, const void* widths, int continuous, double an,
                         void* out, void* stream);
 
+/* Exact adjoint of the linear operator of cagr_apply_weighted (node centres
+ * fixed by `weights` [n]): out = A^T g, where A is `op` with the tree's points
+ * as queries.  op in {A, A_CPU, A21..A23, A3, A2}; g [n,out_dim(op)] and out
+ * [n,3] in `dtype`, original order.  No counterpart in the reference, whose
+ * A2Tk is the adjoint of the direction-reversed operator (A2Tk(An) =
+ * (-A2k(-An))^T), not of A2k; this is the backward of the differentiable
+ * form and the adjoint of the CGNR solve. */
+int cagr_apply_adjoint(cagr_tree* tree, int op, int precision, const void* g, int dtype,
+                       const void* weights, const void* widths, int continuous, double an,
+                       void* out, void* stream);
+
 /* Operator at external queries [nq,3] (qdtype) -- e.g. the GaussRecon sample
  * isovalue pass (main_GaussRecon_cuda.cu:450-467).  widths [nq] (dtype) or
  * NULL for the scalar `width`.  Queries are Morton-ordered internally for the
@@ -140,6 +151,17 @@ int cagr_grid_query(cagr_tree* tree, int op, int precision, const void* attrs, i
  * opening sets of the op. */
 int cagr_count(cagr_tree* tree, int op, const void* attrs, int dtype, const void* weights,
                int64_t* n_far, int64_t* n_near, int64_t* n_visit, void* stream);
+
+/* Opt-in timing of the library's own launches on the caller's stream (no
+ * counterpart in the reference, which records no timings).  enable = 1 clears
+ * and starts recording a HIP event pair around every traversal launch
+ * (kind 0) and every upward pass incl. source records (kind 1) of this handle;
+ * enable = 0 stops.  cagr_profile_read synchronises on the recorded events and
+ * returns the number of recorded launches, their summed duration in ms and,
+ * if each_ms != NULL, the first `cap` durations in launch order (HOST). */
+int cagr_profile(cagr_tree* tree, int enable);
+int cagr_profile_read(cagr_tree* tree, int kind, int64_t* launches, double* total_ms,
+                      double* each_ms, int64_t cap);
 
 #ifdef __cplusplus
 }

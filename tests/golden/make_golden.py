@@ -1,0 +1,96 @@
+"""Writes the golden fixtures in tests/golden/ (run once, in a container that
+has the reference tree; the fixtures are committed, the reference is not).
+
+* ref_core.npz -- vectors produced by the reference's OWN torch-free CPU core
+  (CUDA-based/ext/wn_treecode/wn_treecode_cpu/*.cpp compiled into
+  oracle/_ref/libcagr_ref.so by oracle/Makefile, driven by
+  oracle/ref_harness.cpp): for three small seeded clouds and leaf sizes 1 and
+  8, the 7 serialised tree arrays, the node moments of a vector and a scalar
+  attribute, and multiply_by_A (CPU sign, continuous off/on), multiply_by_AT
+  and multiply_by_G.
+* bunny.npz -- the reference-held end-to-end fixture: the 5000 input points
+  of CUDA-based/input/bunny.xyz and the normals main_CAGR.py produced for
+  them, CUDA-based/results/bunny.xyz (columns 3..5).
+
+Usage:  python tests/golden/make_golden.py [--ref /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from cagr_amd import synth  # noqa: E402
+
+CLOUDS = {
+    "sphere": lambda: synth.normalize(synth.sphere(600, 11)),
+    "torus": lambda: synth.normalize(synth.torus(800, 12)),
+    "clusters": lambda: np.repeat(np.random.default_rng(13).uniform(-0.9, 0.9, (40, 3)), 12, 0),
+}
+LEAVES = (1, 8)
+WIDTH = 0.01
+
+
+def ref_core_vectors():
+    out = {}
+    for name, gen in CLOUDS.items():
+        p = gen().astype(np.float32).astype(np.float64)
+        n = p.shape[0]
+        rng = np.random.default_rng(len(name))
+        mu = synth.random_normals(n, 100 + len(name))
+        s = rng.normal(size=(n, 1))
+        out[f"{name}/points"] = p
+        out[f"{name}/mu"] = mu
+        out[f"{name}/s"] = s
+        for leaf in LEAVES:
+            t = oracle.ref_build_tree(p, 15, leaf)
+            key = f"{name}/leaf{leaf}"
+            for field, arr in zip(("parent", "children", "is_leaf", "half_w", "num_points",
+                                   "node2point_index", "node2point_indexstart"), t.arrays()):
+                out[f"{key}/{field}"] = arr
+            wmu = oracle.point_weights(mu)
+            ws = oracle.point_weights(s)
+            mm = oracle.ref_upward(t, p, mu, wmu)
+            ms = oracle.ref_upward(t, p, s, ws)
+            for field, arr in zip(("attrs", "reppoints", "weights"), mm):
+                out[f"{key}/mu_{field}"] = arr
+            for field, arr in zip(("attrs", "reppoints", "weights"), ms):
+                out[f"{key}/s_{field}"] = arr
+            out[f"{key}/A_cpu"] = oracle.ref_apply("A_cpu", t, p, mu, WIDTH, mm)
+            out[f"{key}/A_cpu_cont"] = oracle.ref_apply("A_cpu", t, p, mu, WIDTH, mm, continuous=True)
+            out[f"{key}/AT"] = oracle.ref_apply("AT", t, p, s, WIDTH, ms)
+            out[f"{key}/G"] = oracle.ref_apply("G", t, p, mu, WIDTH, mm)
+    out["width"] = np.array(WIDTH)
+    return out
+
+
+def bunny(ref_root):
+    base = os.path.join(ref_root, "CUDA-based")
+    pin = np.loadtxt(os.path.join(base, "input", "bunny.xyz"))
+    pout = np.loadtxt(os.path.join(base, "results", "bunny.xyz"))
+    assert np.array_equal(pin[:, :3], pout[:, :3])
+    return {"points": pin[:, :3], "normals": pout[:, 3:6]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    args = ap.parse_args()
+    oracle.build()
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/libcagr_ref.so is missing (make -C oracle needs the reference tree)")
+    np.savez_compressed(os.path.join(HERE, "ref_core.npz"), **ref_core_vectors())
+    np.savez_compressed(os.path.join(HERE, "bunny.npz"), **bunny(args.ref))
+    for f in ("ref_core.npz", "bunny.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

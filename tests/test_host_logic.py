@@ -1,0 +1,90 @@
+"""CPU replay of the HIP tree build (tests/host_logic/build_logic.cpp runs the
+per-element bodies of cagr_amd/csrc/tree_logic.h serially, in the kernels'
+order) against the oracle tree, including degenerate inputs; compiled with
+AddressSanitizer / UBSan on the host so an out-of-bounds index in the shared
+logic fails here rather than on a GPU."""
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from cagr_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def hl(tmp_path_factory):
+    so = str(tmp_path_factory.mktemp("hl") / "libhl.so")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared", "-fsanitize=undefined",
+                    "-fno-sanitize-recover=all", "-o", so,
+                    os.path.join(HERE, "host_logic", "build_logic.cpp")], check=True)
+    L = ct.CDLL(so)
+    P, I = ct.c_void_p, ct.c_int64
+    L.hl_build.argtypes = [P, I, ct.c_int, ct.c_int, ct.c_double, P, P, P]
+    L.hl_build.restype = I
+    L.hl_export.argtypes = [P, I, P, I, P] + [P] * 7
+    L.hl_export.restype = I
+    return L
+
+
+def hl_tree(L, p, md, S, theta=0.5):
+    p = np.ascontiguousarray(p, np.float64)
+    n = p.shape[0]
+    P = ct.c_void_p
+    m = np.zeros(1, np.int64)
+    L.hl_build(p.ctypes.data_as(P), n, md, S, theta, m.ctypes.data_as(P), None, None)
+    M = int(m[0])
+    topo = np.zeros((max(M, 1), 32), np.uint8)
+    perm = np.zeros(max(n, 1), np.int32)
+    L.hl_build(p.ctypes.data_as(P), n, md, S, theta, m.ctypes.data_as(P),
+               topo.ctypes.data_as(P), perm.ctypes.data_as(P))
+    t = oracle._alloc_tree(M, 0)
+    ll = L.hl_export(p.ctypes.data_as(P), n, topo.ctypes.data_as(P), M, perm.ctypes.data_as(P),
+                     *[a.ctypes.data_as(P) for a in t.arrays()[:5]], None,
+                     t.node2point_indexstart.ctypes.data_as(P))
+    t.node2point_index = np.zeros(ll, np.int64)
+    L.hl_export(p.ctypes.data_as(P), n, topo.ctypes.data_as(P), M, perm.ctypes.data_as(P),
+                *[a.ctypes.data_as(P) for a in t.arrays()])
+    thr2 = topo[:M].view(np.float64)[:, 0]
+    return t, thr2
+
+
+def _cases():
+    rng = np.random.default_rng(1)
+    bad = rng.uniform(-1, 1, (500, 3))
+    bad[::37, 1] = np.nan
+    bad[5] = [0, 0, 0]
+    bad[6] = [-0.0, 1.0, -1.0]
+    bad[7] = [-1e-300, 1e-300, 0.5]
+    bad[8] = [np.inf, -np.inf, 2.0]
+    return {
+        "sphere": synth.normalize(synth.sphere(4000, 1)).astype(np.float32).astype(np.float64),
+        "torus": synth.normalize(synth.torus(3000, 2)),
+        "duplicates": np.repeat(rng.uniform(-1, 1, (50, 3)), 40, axis=0),
+        "outside_box": rng.uniform(-2, 2, (3000, 3)),
+        "nan_and_edges": bad,
+        "three": rng.uniform(-1, 1, (3, 3)),
+        "one": rng.uniform(-1, 1, (1, 3)),
+    }
+
+
+@pytest.mark.parametrize("name", list(_cases()))
+@pytest.mark.parametrize("S", [1, 16, 32, 64])
+@pytest.mark.parametrize("md", [15, 6, 0])
+def test_kernel_logic_replay_matches_oracle_tree(hl, name, S, md):
+    p = _cases()[name]
+    a = oracle.build_tree(p, md, S)
+    b, thr2 = hl_tree(hl, p, md, S)
+    for x, y in zip(a.arrays(), b.arrays()):
+        assert np.array_equal(x, y)
+    # opening thresholds: (2/theta * 2^-depth)^2 with theta = 0.5
+    assert np.array_equal(thr2, (4.0 * a.half_w) ** 2)
+
+
+def test_empty_cloud(hl):
+    b, _ = hl_tree(hl, np.zeros((0, 3)), 15, 1)
+    assert b.num_nodes == 0
