@@ -31,8 +31,8 @@ EXPORTS = (
     "cagr_last_error", "cagr_version", "cagr_tree_build", "cagr_tree_destroy", "cagr_tree_info",
     "cagr_tree_export", "cagr_tree_permutation", "cagr_upward", "cagr_apply",
     "cagr_apply_weighted", "cagr_apply_adjoint", "cagr_apply_queries", "cagr_grid_query",
-    "cagr_count",
-    "cagr_profile", "cagr_profile_read",
+    "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
+    "cagr_order_items", "cagr_profile", "cagr_profile_read",
 )
 
 _P, _I, _D, _C = ct.c_void_p, ct.c_int64, ct.c_double, ct.c_int
@@ -51,6 +51,10 @@ _SIGS = {
     "cagr_apply_queries": (_C, [_P, _C, _C, _P, _C, _P, _C, _I, _P, _D, _C, _D, _P, _P]),
     "cagr_grid_query": (_C, [_P, _C, _C, _P, _C, _P, _P, _P, _D, _C, _D, _P, _P]),
     "cagr_count": (_C, [_P, _C, _P, _C, _P, _P, _P, _P, _P]),
+    "cagr_forest_build": (_C, [_P, _P, _I, _C, _C, _C, _D, _P, ct.POINTER(_P)]),
+    "cagr_forest_roots": (_C, [_P, _P]),
+    "cagr_set_launch": (_C, [_P, _C, _C, _I]),
+    "cagr_order_items": (_C, [_P, _P]),
     "cagr_profile": (_C, [_P, _C]),
     "cagr_profile_read": (_C, [_P, _C, _P, _P, _P, _I]),
 }

@@ -2,6 +2,7 @@
 // mapping, and the calls into the build / upward / traversal code.
 #include "common.h"
 #include "tree_export.h"
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -159,6 +160,80 @@ int cagr_profile_read(cagr_tree* t, int kind, int64_t* launches, double* total_m
     return CAGR_OK;
 }
 
+int cagr_forest_build(const void* points, const int64_t* cloud_offsets, int64_t num_clouds,
+                      int dtype, int max_depth, int leaf_size, double theta, void* stream,
+                      cagr_tree** out) {
+    CAGR_REQUIRE(out != nullptr && cloud_offsets != nullptr, CAGR_ERR_ARG, "null argument");
+    *out = nullptr;
+    CAGR_REQUIRE(num_clouds >= 1 && num_clouds < (int64_t(1) << 18), CAGR_ERR_ARG,
+                 "num_clouds must be in [1, 2^18)");
+    CAGR_REQUIRE(cloud_offsets[0] == 0, CAGR_ERR_ARG, "cloud_offsets[0] must be 0");
+    for (int64_t c = 0; c < num_clouds; ++c)
+        CAGR_REQUIRE(cloud_offsets[c + 1] >= cloud_offsets[c], CAGR_ERR_ARG,
+                     "cloud_offsets must be non-decreasing");
+    const int64_t n = cloud_offsets[num_clouds];
+    CAGR_REQUIRE(dtype == CAGR_F32 || dtype == CAGR_F64, CAGR_ERR_ARG, "bad points dtype");
+    CAGR_REQUIRE(n >= 0 && n < (int64_t(1) << 31) - 1, CAGR_ERR_ARG, "too many points");
+    CAGR_REQUIRE(n == 0 || points != nullptr, CAGR_ERR_ARG, "null points");
+    CAGR_REQUIRE(max_depth >= 0 && max_depth <= kLevels, CAGR_ERR_ARG, "max_depth must be in [0, 15]");
+    CAGR_REQUIRE(leaf_size >= 1 && leaf_size <= 64, CAGR_ERR_ARG, "leaf_size must be in [1, 64]");
+    CAGR_REQUIRE(theta > 0.0 && theta <= 2.0, CAGR_ERR_ARG, "theta must be in (0, 2]");
+    cagr_tree* t = new (std::nothrow) cagr_tree;
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_STATE, "out of host memory");
+    const int rc = build_tree(t, points, n, dtype, max_depth, leaf_size, theta, (hipStream_t)stream,
+                              cloud_offsets, num_clouds);
+    if (rc) {
+        const std::string keep = g_err;
+        cagr_tree_destroy(t);
+        g_err = keep;
+        return rc;
+    }
+    *out = t;
+    return CAGR_OK;
+}
+
+int cagr_forest_roots(const cagr_tree* t, int64_t* node_offsets) {
+    CAGR_REQUIRE(t != nullptr && node_offsets != nullptr, CAGR_ERR_ARG, "null argument");
+    for (int64_t c = 0; c <= t->num_clouds; ++c) node_offsets[c] = t->cloud_node[c];
+    return CAGR_OK;
+}
+
+int cagr_set_launch(cagr_tree* t, int persistent, int waves_per_block, int64_t grid_blocks) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    CAGR_REQUIRE(waves_per_block >= 1 && waves_per_block <= 8, CAGR_ERR_ARG,
+                 "waves_per_block must be in [1, 8]");
+    CAGR_REQUIRE(grid_blocks >= 0, CAGR_ERR_ARG, "grid_blocks must be >= 0");
+    t->persistent = persistent != 0;
+    t->waves_per_block = waves_per_block;
+    t->grid_blocks = grid_blocks;
+    return CAGR_OK;
+}
+
+int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    if (t->num_items == 0) return CAGR_OK;
+    if (!query_cost) {   // back to natural order
+        (void)hipFree(t->item_order);
+        t->item_order = nullptr;
+        return CAGR_OK;
+    }
+    std::vector<int4> items(t->num_items);
+    CAGR_HIP(hipMemcpy(items.data(), t->items, t->num_items * sizeof(int4), hipMemcpyDeviceToHost));
+    std::vector<int32_t> perm(t->n);
+    CAGR_HIP(hipMemcpy(perm.data(), t->perm, t->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int64_t> cost(t->num_items, 0);
+    for (int64_t i = 0; i < t->num_items; ++i)
+        for (int k = 0; k < items[i].w; ++k) cost[i] += query_cost[perm[items[i].z + k]];
+    std::vector<int32_t> order(t->num_items);
+    for (int64_t i = 0; i < t->num_items; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    if (!t->item_order) CAGR_HIP(hipMalloc(&t->item_order, t->num_items * sizeof(int32_t)));
+    CAGR_HIP(hipMemcpy(t->item_order, order.data(), t->num_items * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+    return CAGR_OK;
+}
+
 int cagr_tree_destroy(cagr_tree* t) {
     if (!t) return CAGR_OK;
     for (auto& ev : t->prof_events)
@@ -176,6 +251,9 @@ int cagr_tree_destroy(cagr_tree* t) {
     (void)hipFree(t->m64);
     (void)hipFree(t->qws);
     (void)hipFree(t->leaf_of);
+    (void)hipFree(t->items);
+    (void)hipFree(t->item_order);
+    (void)hipFree(t->item_counter);
     (void)hipFree(t->adj_node);
     (void)hipFree(t->adj_pt);
     delete t;
@@ -269,6 +347,7 @@ int cagr_apply_adjoint(cagr_tree* t, int op, int precision, const void* g, int d
     CAGR_REQUIRE(op == CAGR_OP_A || op == CAGR_OP_A_CPU || op == CAGR_OP_A3 || op == CAGR_OP_A2 ||
                      (op >= CAGR_OP_A21 && op <= CAGR_OP_A23),
                  CAGR_ERR_ARG, "adjoint supports A, A_CPU, A21..A23, A2 and A3");
+    CAGR_REQUIRE(t->num_clouds == 1, CAGR_ERR_ARG, "the adjoint needs a single-cloud tree");
     if (t->n == 0) return CAGR_OK;
     CAGR_REQUIRE(g && weights && widths && out, CAGR_ERR_ARG, "null g / weights / widths / out");
     hipStream_t st = (hipStream_t)stream;

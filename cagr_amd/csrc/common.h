@@ -93,6 +93,20 @@ struct cagr_tree {
     size_t ws_queries = 0;        // grow-only query workspace (explicit-query sorting)
     void* qws = nullptr;
 
+    // clouds of a forest (a plain tree is one cloud): sorted point ranges and
+    // node id ranges [cloud_node[c], cloud_node[c+1]) (host)
+    int64_t num_clouds = 1;
+    std::vector<int64_t> cloud_pt, cloud_node;
+    // wave work items of the persistent traversal (device): {root, stop, q0, qn}
+    int4* items = nullptr;
+    int32_t* item_order = nullptr;   // execution order of the items
+    int32_t* item_counter = nullptr;
+    int64_t num_items = 0;
+    int persistent = 0;              // 1: traversal through the item queue
+    int waves_per_block = 4;         // launch shape of the traversal kernels
+    int num_cus = 256;               // compute units of the device
+    int64_t grid_blocks = 0;         // persistent grid size (0 = from the device)
+
     // opt-in launch timing (cagr_profile): event pairs per kind, reused
     bool profiling = false;
     std::vector<hipEvent_t> prof_events[2];
@@ -107,7 +121,8 @@ int prof_mark(cagr_tree* t, int kind, hipStream_t st);
 namespace cagr {
 // implemented in tree_build.hip
 int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_depth,
-               int leaf_size, double theta, hipStream_t st);
+               int leaf_size, double theta, hipStream_t st, const int64_t* cloud_off = nullptr,
+               int64_t nclouds = 1);
 int sort_pairs_u64(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp,
                    uint32_t* hist, int64_t n, int bits, hipStream_t st);
 int64_t sort_hist_len(int64_t n);

@@ -76,16 +76,16 @@ __device__ inline void store_out(const TP& p, int64_t i, double v) {
 
 template <int QM>
 __device__ inline bool load_query(const TP& p, const double* __restrict__ pts64, int64_t idx,
-                                  double x[3], double& w, int64_t& oi) {
+                                  int64_t qend, double x[3], double& w, int64_t& oi) {
     bool valid;
     if constexpr (QM == QM_SELF) {
-        valid = idx < p.nq;
+        valid = idx < qend;
         const int64_t s = valid ? idx : 0;
         x[0] = pts64[3 * s]; x[1] = pts64[3 * s + 1]; x[2] = pts64[3 * s + 2];
         oi = p.perm[s];
         w = p.widths ? load_as_double(p.widths, p.wdtype, oi) : p.width;
     } else if constexpr (QM == QM_EXPLICIT) {
-        valid = idx < p.nq;
+        valid = idx < qend;
         const int64_t s = valid ? idx : 0;
         const int64_t i = p.qorder ? (int64_t)p.qorder[s] : s;
         for (int d = 0; d < 3; ++d) x[d] = load_as_double(p.q, p.qdtype, 3 * i + d);
@@ -187,21 +187,20 @@ struct VAcc64 {
     }
 };
 
+// one wave: queries idx0 + lane (< qend), tree nodes [root, stop)
 template <int P, int VOP, int QM, bool CONT, bool CNT>
-__global__ __launch_bounds__(kThreads) void k_trav_v(
-    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+__device__ __forceinline__ void trav_v(
+    const TP& p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
-    const float* __restrict__ pts32) {
+    const float* __restrict__ pts32, int64_t idx0, int64_t qend, int32_t root, int32_t stop) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave =
-        xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    const int64_t idx = wave * 64 + lane;
+    const int64_t idx = idx0 + lane;
     double x[3], w;
     int64_t oi;
-    const bool valid = load_query<QM>(p, pts64, idx, x, w, oi);
+    const bool valid = load_query<QM>(p, pts64, idx, qend, x, w, oi);
     if (__ballot(valid) == 0) return;
 
-    int32_t resume = valid ? p.root : INT_MAX;
+    int32_t resume = valid ? root : INT_MAX;
     int64_t nfar = 0, nnear = 0, nvisit = 0;
 
     if constexpr (P == CAGR_PREC_FP32C) {
@@ -218,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void k_trav_v(
         const float w2 = wf * wf;
         VAcc32<VOP, CONT> acc;
         acc.init();
-        for (int32_t n = p.root; n < p.stop;) {
+        for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = topo[n];
             const NodeV32 nr = nodes[n];
@@ -296,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void k_trav_v(
         const SrcV64* __restrict__ src = (const SrcV64*)src_;
         VAcc64<VOP, CONT> acc;
         acc.init();
-        for (int32_t n = p.root; n < p.stop;) {
+        for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = topo[n];
             const NodeV64 nr = nodes[n];
@@ -367,21 +366,19 @@ __global__ __launch_bounds__(kThreads) void k_trav_v(
 // scalar family: AT (an = 0), A2Tk (K = 1, direction kdir), sum_k A2Tk (K = 3)
 // ============================================================================
 template <int P, int K, int QM, bool CNT>
-__global__ __launch_bounds__(kThreads) void k_trav_s(
-    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+__device__ __forceinline__ void trav_s(
+    const TP& p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
-    const float* __restrict__ pts32) {
+    const float* __restrict__ pts32, int64_t idx0, int64_t qend, int32_t root, int32_t stop) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave =
-        xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    const int64_t idx = wave * 64 + lane;
+    const int64_t idx = idx0 + lane;
     double x[3], w;
     int64_t oi;
-    const bool valid = load_query<QM>(p, pts64, idx, x, w, oi);
+    const bool valid = load_query<QM>(p, pts64, idx, qend, x, w, oi);
     if (__ballot(valid) == 0) return;
 
     int32_t resume[K];
-    for (int k = 0; k < K; ++k) resume[k] = valid ? p.root : INT_MAX;
+    for (int k = 0; k < K; ++k) resume[k] = valid ? root : INT_MAX;
     int64_t nfar = 0, nnear = 0, nvisit = 0;
     const double han = 0.5 * p.an;
 
@@ -402,7 +399,7 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
         const float w2 = wf * wf;
         float s[3] = {0.f, 0.f, 0.f}, c[3] = {0.f, 0.f, 0.f}, lp[3] = {0.f, 0.f, 0.f},
               fp[3] = {0.f, 0.f, 0.f};
-        for (int32_t n = p.root; n < p.stop;) {
+        for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = topo[n];
             bool nearl[K];
@@ -501,7 +498,7 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
                 for (int cc = 0; cc < 3; ++cc) acc[cc] += gi3 * d[cc] + (cc == dir ? hg : 0.0);
             }
         };
-        for (int32_t n = p.root; n < p.stop;) {
+        for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = topo[n];
             bool nearl[K];
@@ -554,6 +551,71 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
     }
 }
 
+// ---------------------------------------------------------------- kernels
+// one wave per 64 consecutive queries, every wave on the whole tree
+template <int P, int VOP, int QM, bool CONT, bool CNT>
+__global__ __launch_bounds__(kThreads) void k_trav_v(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32) {
+    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    trav_v<P, VOP, QM, CONT, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root,
+                                  p.stop);
+}
+
+template <int P, int K, int QM, bool CNT>
+__global__ __launch_bounds__(kThreads) void k_trav_s(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32) {
+    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    trav_s<P, K, QM, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root, p.stop);
+}
+
+// Persistent form: a grid sized to the device pulls wave items {root, stop,
+// q0, qn} (a cloud's tree and up to 64 of its sorted points) from an atomic
+// queue in `order` (longest first when the autotuner has costed them) until
+// the queue is empty.  Any block shape up to 512 threads.
+struct Items {
+    const int4* items;
+    const int32_t* order;
+    int32_t count;
+    int32_t* counter;
+};
+
+__device__ __forceinline__ bool next_item(const Items& q, int4& item) {
+    int it = 0;
+    if ((threadIdx.x & 63) == 0) it = atomicAdd(q.counter, 1);
+    it = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(it, 0));
+    if (it >= q.count) return false;
+    item = q.items[q.order ? q.order[it] : it];
+    return true;
+}
+
+template <int P, int VOP, bool CONT, bool CNT>
+__global__ __launch_bounds__(512) void k_trav_v_items(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32, Items q) {
+    int4 it;
+    while (next_item(q, it))
+        trav_v<P, VOP, QM_SELF, CONT, CNT>(p, topo, nodes_, src_, pts64, pts32, it.z, it.z + it.w,
+                                           it.x, it.y);
+}
+
+template <int P, int K, bool CNT>
+__global__ __launch_bounds__(512) void k_trav_s_items(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32, Items q) {
+    int4 it;
+    while (next_item(q, it))
+        trav_s<P, K, QM_SELF, CNT>(p, topo, nodes_, src_, pts64, pts32, it.z, it.z + it.w, it.x,
+                                   it.y);
+}
+
+__global__ void k_reset_counter(int32_t* c) { *c = 0; }
+
 // ---------------------------------------------------------------- dispatch
 template <int P, int VOP, int QM, bool CNT>
 int launch_v(const TP& tp, cagr_tree* t, int continuous, int64_t blocks, hipStream_t st) {
@@ -582,6 +644,61 @@ template <int P, int K, int QM, bool CNT>
 int launch_s(const TP& tp, cagr_tree* t, int64_t blocks, hipStream_t st) {
     k_trav_s<P, K, QM, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src, t->pts64,
                                                          t->pts32);
+    CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+
+template <int P, bool CNT>
+int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t st) {
+    const Items q{t->items, t->item_order, (int32_t)t->num_items, t->item_counter};
+    k_reset_counter<<<1, 1, 0, st>>>(t->item_counter);
+    CAGR_LAUNCHED();
+    const int threads = 64 * t->waves_per_block;
+    // default: enough blocks for 8 waves per SIMD on every CU
+    const int64_t blocks = t->grid_blocks > 0 ? t->grid_blocks
+                                              : (int64_t)t->num_cus * 32 / t->waves_per_block;
+    const auto* topo = t->topo;
+    const void* nodes = t->nodes;
+    const void* src = t->src;
+    switch (op) {
+        case CAGR_OP_A:
+        case CAGR_OP_A_CPU:
+            if (continuous)
+                k_trav_v_items<P, VOP_A, true, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            else
+                k_trav_v_items<P, VOP_A, false, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            break;
+        case CAGR_OP_G:
+            k_trav_v_items<P, VOP_G, false, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            break;
+        case CAGR_OP_A21:
+        case CAGR_OP_A22:
+        case CAGR_OP_A23:
+        case CAGR_OP_A2:
+            if (continuous)
+                k_trav_v_items<P, VOP_A2, true, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            else
+                k_trav_v_items<P, VOP_A2, false, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            break;
+        case CAGR_OP_A3:
+            if (continuous)
+                k_trav_v_items<P, VOP_A3, true, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            else
+                k_trav_v_items<P, VOP_A3, false, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            break;
+        case CAGR_OP_AT:
+        case CAGR_OP_A2T1:
+        case CAGR_OP_A2T2:
+        case CAGR_OP_A2T3:
+            k_trav_s_items<P, 1, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            break;
+        case CAGR_OP_A2T_SUM:
+            k_trav_s_items<P, 3, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            break;
+        default:
+            set_error("unknown op");
+            return CAGR_ERR_ARG;
+    }
     CAGR_LAUNCHED();
     return CAGR_OK;
 }
@@ -657,6 +774,13 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     const int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
     const bool cnt = cnt_far || cnt_near || cnt_visit;
     const bool p64 = precision == CAGR_PREC_FP64;
+    if (q.mode == QM_SELF && (t->persistent || t->num_clouds > 1)) {
+        if (cnt) return launch_items<CAGR_PREC_FP64, true>(tp, t, op, continuous, st);
+        if (p64) return launch_items<CAGR_PREC_FP64, false>(tp, t, op, continuous, st);
+        return launch_items<CAGR_PREC_FP32C, false>(tp, t, op, continuous, st);
+    }
+    CAGR_REQUIRE(t->num_clouds == 1, CAGR_ERR_ARG,
+                 "external and grid queries need a single-cloud tree, not a forest");
     int rc;
 #define CAGR_LAUNCH_QM(QMV)                                                                   \
     if (cnt) rc = launch_all<CAGR_PREC_FP64, QMV, true>(tp, t, op, continuous, blocks, st);    \

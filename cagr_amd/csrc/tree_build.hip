@@ -20,15 +20,34 @@ constexpr int kThreads = 256;
 inline int64_t blocks_for(int64_t n, int per = kThreads) { return (n + per - 1) / per; }
 
 // ---------------------------------------------------------------- keys
+// coff (device, [nclouds + 1]) may be null for a single cloud
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_keys(const T* __restrict__ pts, int64_t n,
-                                                   uint64_t* __restrict__ keys,
+                                                   const int64_t* __restrict__ coff,
+                                                   int64_t nclouds, uint64_t* __restrict__ keys,
                                                    int32_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    keys[i] = morton_key(cell_index(pts[3 * i]), cell_index(pts[3 * i + 1]),
-                         cell_index(pts[3 * i + 2]));
+    uint64_t cloud = 0;
+    if (coff) {   // last c with coff[c] <= i
+        int64_t lo = 0, hi = nclouds - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) / 2;
+            if (coff[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        cloud = (uint64_t)lo;
+    }
+    keys[i] = forest_key(cloud, morton_key(cell_index(pts[3 * i]), cell_index(pts[3 * i + 1]),
+                                           cell_index(pts[3 * i + 2])));
     idx[i] = (int32_t)i;
+}
+
+// node id range [roots[c], roots[c+1]) of every cloud
+__global__ void k_roots(const int64_t* __restrict__ coff, int64_t nclouds,
+                        const int32_t* __restrict__ off, int32_t* __restrict__ roots) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= nclouds) roots[c] = off[coff[c]];
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -256,15 +275,16 @@ int morton_order(const void* pts, int dtype, int64_t n, int32_t* order, void* sc
     uint32_t* hist = (uint32_t*)p;
     const int64_t nb = blocks_for(n);
     if (dtype == CAGR_F32)
-        k_keys<float><<<nb, kThreads, 0, st>>>((const float*)pts, n, keys, order);
+        k_keys<float><<<nb, kThreads, 0, st>>>((const float*)pts, n, nullptr, 1, keys, order);
     else
-        k_keys<double><<<nb, kThreads, 0, st>>>((const double*)pts, n, keys, order);
+        k_keys<double><<<nb, kThreads, 0, st>>>((const double*)pts, n, nullptr, 1, keys, order);
     CAGR_LAUNCHED();
     return sort_pairs_u64(keys, order, ktmp, vtmp, hist, n, kKeyBits, st);
 }
 
 int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_depth,
-               int leaf_size, double theta, hipStream_t st) {
+               int leaf_size, double theta, hipStream_t st, const int64_t* cloud_off,
+               int64_t nclouds) {
     t->n = n;
     t->max_depth = max_depth;
     t->leaf_size = leaf_size;
@@ -272,6 +292,14 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     t->open_factor = 2.0 / theta;
     t->point_dtype = dtype;
     CAGR_HIP(hipGetDevice(&t->device));
+    // clouds: a plain tree is a forest of one
+    t->num_clouds = cloud_off ? nclouds : 1;
+    t->cloud_pt.assign(t->num_clouds + 1, 0);
+    if (cloud_off)
+        for (int64_t c = 0; c <= nclouds; ++c) t->cloud_pt[c] = cloud_off[c];
+    else
+        t->cloud_pt[1] = n;
+    t->cloud_node.assign(t->num_clouds + 1, 0);
     if (n == 0) {   // the reference builds no node at all for an empty cloud
         t->m = 0;
         t->group_off.assign(kGroups + 1, 0);
@@ -301,13 +329,22 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     CAGR_HIP(hipMalloc(&sd, n));
     CAGR_HIP(hipMalloc(&gcnt, (2 * kGroups + 1) * sizeof(int32_t)));
 
+    int64_t* coff_d = nullptr;
+    struct Guard2 { int64_t** p; ~Guard2() { (void)hipFree(*p); } } guard2{&coff_d};
+    int cloud_bits = 0;
+    if (cloud_off) {
+        CAGR_HIP(hipMalloc(&coff_d, (nclouds + 1) * sizeof(int64_t)));
+        CAGR_HIP(hipMemcpyAsync(coff_d, t->cloud_pt.data(), (nclouds + 1) * sizeof(int64_t),
+                                hipMemcpyHostToDevice, st));
+        while ((int64_t(1) << cloud_bits) < nclouds) ++cloud_bits;
+    }
     const int64_t nb = blocks_for(n);
     if (dtype == CAGR_F32)
-        k_keys<float><<<nb, kThreads, 0, st>>>((const float*)points, n, t->keys, t->perm);
+        k_keys<float><<<nb, kThreads, 0, st>>>((const float*)points, n, coff_d, nclouds, t->keys, t->perm);
     else
-        k_keys<double><<<nb, kThreads, 0, st>>>((const double*)points, n, t->keys, t->perm);
+        k_keys<double><<<nb, kThreads, 0, st>>>((const double*)points, n, coff_d, nclouds, t->keys, t->perm);
     CAGR_LAUNCHED();
-    int rc = sort_pairs_u64(t->keys, t->perm, ktmp, vtmp, hist, n, kKeyBits, st);
+    int rc = sort_pairs_u64(t->keys, t->perm, ktmp, vtmp, hist, n, kKeyBits + cloud_bits, st);
     if (rc) return rc;
 
     k_chain<<<blocks_for(n + 1), kThreads, 0, st>>>(t->keys, n, leaf_size, max_depth, ld, sd, cnt);
@@ -320,6 +357,21 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     const int64_t m = m32;
     t->m = m;
     CAGR_REQUIRE(m > 0, CAGR_ERR_STATE, "tree build produced no nodes");
+    if (cloud_off) {
+        int32_t* roots_d = nullptr;
+        CAGR_HIP(hipMalloc(&roots_d, (nclouds + 1) * sizeof(int32_t)));
+        k_roots<<<(int)((nclouds + 256) / 256), 256, 0, st>>>(coff_d, nclouds, cnt, roots_d);
+        std::vector<int32_t> roots(nclouds + 1);
+        const hipError_t e1 = hipMemcpyAsync(roots.data(), roots_d, (nclouds + 1) * sizeof(int32_t),
+                                             hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipFree(roots_d);
+        CAGR_HIP(e1);
+        CAGR_HIP(e2);
+        for (int64_t c = 0; c <= nclouds; ++c) t->cloud_node[c] = roots[c];
+    } else {
+        t->cloud_node[1] = m;
+    }
 
     CAGR_HIP(hipMalloc(&t->topo, m * sizeof(TopoRec)));
     CAGR_HIP(hipMalloc(&t->order, m * sizeof(int32_t)));
@@ -350,6 +402,26 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     CAGR_LAUNCHED();
     t->n_leaves = hc[0];
     t->depth = hdepth;
+
+    // wave work items: per cloud, its node range and its points 64 at a time
+    {
+        std::vector<int4> items;
+        for (int64_t c = 0; c < t->num_clouds; ++c)
+            for (int64_t q0 = t->cloud_pt[c]; q0 < t->cloud_pt[c + 1]; q0 += 64) {
+                const int64_t qn = std::min<int64_t>(64, t->cloud_pt[c + 1] - q0);
+                items.push_back(make_int4((int)t->cloud_node[c], (int)t->cloud_node[c + 1], (int)q0,
+                                          (int)qn));
+            }
+        t->num_items = (int64_t)items.size();
+        CAGR_HIP(hipMalloc(&t->items, items.size() * sizeof(int4)));
+        CAGR_HIP(hipMalloc(&t->item_counter, sizeof(int32_t)));
+        CAGR_HIP(hipMemcpyAsync(t->items, items.data(), items.size() * sizeof(int4),
+                                hipMemcpyHostToDevice, st));
+        CAGR_HIP(hipStreamSynchronize(st));
+        int cus = 0;
+        CAGR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, t->device));
+        t->num_cus = cus > 0 ? cus : 256;
+    }
 
     // workspace sized for every operator family
     CAGR_HIP(hipMalloc(&t->src, n * 64));

@@ -21,7 +21,8 @@
 // i.e. ix = 0).  Interleaving the three cell indices, octant bits (z,y,x) per
 // level from the root down, gives a 45-bit key whose lexicographic order is
 // the preorder of leaves.  With the points stably sorted by key:
-//   * cpl(a,b)   = number of leading levels on which two keys agree (0..15);
+//   * cpl(a,b)   = number of leading levels on which two keys agree (0..15;
+//     -1 for keys of different clouds of a forest);
 //   * a point b's depth-d cell holds > S points iff some window of S+1
 //     consecutive sorted points containing b has cpl >= d, so
 //     leaf_depth(b) = min(1 + max_window_cpl(b), max_depth)   (0 if N <= S);
@@ -75,8 +76,17 @@ CAGR_HD inline uint64_t morton_key(uint32_t ix, uint32_t iy, uint32_t iz) {
     return spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
 }
 
-// number of leading levels (3-bit groups from the top) on which a and b agree
+// Keys of a forest (several independent clouds sorted together) carry the
+// cloud id above the 45 Morton bits; points of different clouds share no
+// cell, not even a root.
+CAGR_HD inline uint64_t forest_key(uint64_t cloud, uint64_t morton) {
+    return (cloud << kKeyBits) | morton;
+}
+
+// number of leading levels (3-bit groups from the top) on which a and b agree;
+// -1 if they belong to different clouds
 CAGR_HD inline int cpl(uint64_t a, uint64_t b) {
+    if ((a ^ b) >> kKeyBits) return -1;
     const uint64_t x = (a ^ b) & ((1ull << kKeyBits) - 1);
     if (x == 0) return kLevels;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -113,9 +123,9 @@ CAGR_HD inline int start_depth_of(const uint64_t* keys, int64_t b) {
     return b == 0 ? 0 : cpl(keys[b - 1], keys[b]) + 1;
 }
 
-// one past the last sorted position sharing key[b]'s depth-d cell (d >= 0)
+// one past the last sorted position sharing key[b]'s depth-d cell (d >= 0;
+// depth 0 is the cloud's root cell)
 CAGR_HD inline int64_t cell_end(const uint64_t* keys, int64_t n, int64_t b, int d) {
-    if (d == 0) return n;
     int64_t lo = b + 1, hi = n;     // answer in [b+1, n]
     const uint64_t kb = keys[b];
     while (lo < hi) {
@@ -128,7 +138,6 @@ CAGR_HD inline int64_t cell_end(const uint64_t* keys, int64_t n, int64_t b, int 
 
 // first sorted position sharing key[b]'s depth-d cell
 CAGR_HD inline int64_t cell_begin(const uint64_t* keys, int64_t b, int d) {
-    if (d == 0) return 0;
     int64_t lo = 0, hi = b;         // answer in [0, b]
     const uint64_t kb = keys[b];
     while (lo < hi) {

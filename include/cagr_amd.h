@@ -152,6 +152,29 @@ int cagr_grid_query(cagr_tree* tree, int op, int precision, const void* attrs, i
 int cagr_count(cagr_tree* tree, int op, const void* attrs, int dtype, const void* weights,
                int64_t* n_far, int64_t* n_near, int64_t* n_visit, void* stream);
 
+/* Batched form: one forest over num_clouds independent clouds stored back to
+ * back in points [cloud_offsets[num_clouds], 3] (cloud c = rows
+ * [cloud_offsets[c], cloud_offsets[c+1]), HOST offsets).  Every cloud gets its
+ * own octree (the same tree cagr_tree_build would give it); one upward pass
+ * and one persistent traversal launch serve all of them.  The handle works
+ * with cagr_apply / cagr_apply_weighted / cagr_count / cagr_upward on the
+ * concatenated attrs / widths / out (no counterpart in the reference, which
+ * handles one cloud per process). */
+int cagr_forest_build(const void* points, const int64_t* cloud_offsets, int64_t num_clouds,
+                      int dtype, int max_depth, int leaf_size, double theta, void* stream,
+                      cagr_tree** out);
+/* Node id ranges of the clouds: node_offsets [num_clouds + 1] (HOST). */
+int cagr_forest_roots(const cagr_tree* tree, int64_t* node_offsets);
+/* Launch shape of the traversal: persistent = 1 routes single-cloud trees
+ * through the persistent work-queue kernel too (forests always use it);
+ * waves_per_block in [1, 8]; grid_blocks = 0 sizes the persistent grid to 8
+ * waves per SIMD on every CU. */
+int cagr_set_launch(cagr_tree* tree, int persistent, int waves_per_block, int64_t grid_blocks);
+/* Longest-first order of the persistent work items from a per-query cost
+ * (HOST int64 [n], original order, e.g. n_far + n_near from cagr_count);
+ * NULL restores the natural order. */
+int cagr_order_items(cagr_tree* tree, const int64_t* query_cost);
+
 /* Opt-in timing of the library's own launches on the caller's stream (no
  * counterpart in the reference, which records no timings).  enable = 1 clears
  * and starts recording a HIP event pair around every traversal launch

@@ -88,3 +88,53 @@ def test_kernel_logic_replay_matches_oracle_tree(hl, name, S, md):
 def test_empty_cloud(hl):
     b, _ = hl_tree(hl, np.zeros((0, 3)), 15, 1)
     assert b.num_nodes == 0
+
+
+def test_forest_replay_equals_per_cloud_trees(hl):
+    """Clouds sorted together with the cloud id above the Morton bits form a
+    forest whose per-cloud subtrees are exactly the standalone trees."""
+    P = ct.c_void_p
+    hl.hl_build_forest.argtypes = [P, P, ct.c_int64, ct.c_int, ct.c_int, ct.c_double, P, P, P]
+    hl.hl_build_forest.restype = ct.c_int64
+    rng = np.random.default_rng(4)
+    clouds = [synth.normalize(synth.sphere(900, 1)), np.zeros((0, 3)), rng.uniform(-1, 1, (5, 3)),
+              synth.normalize(synth.torus(1300, 2)), np.repeat(rng.uniform(-1, 1, (3, 3)), 30, 0),
+              rng.uniform(-1, 1, (1, 3))]
+    coff = np.cumsum([0] + [c.shape[0] for c in clouds]).astype(np.int64)
+    pts = np.ascontiguousarray(np.concatenate(clouds), np.float64)
+    for S in (1, 32):
+        m = hl.hl_build_forest(pts.ctypes.data_as(P), coff.ctypes.data_as(P), len(clouds), 15, S, 0.5,
+                               None, None, None)
+        topo = np.zeros((m, 32), np.uint8)
+        perm = np.zeros(pts.shape[0], np.int32)
+        roots = np.zeros(len(clouds) + 1, np.int32)
+        hl.hl_build_forest(pts.ctypes.data_as(P), coff.ctypes.data_as(P), len(clouds), 15, S, 0.5,
+                           topo.ctypes.data_as(P), perm.ctypes.data_as(P), roots.ctypes.data_as(P))
+        ints = topo.view(np.int32).reshape(m, 8)
+        for c, cp in enumerate(clouds):
+            r0, r1 = roots[c], roots[c + 1]
+            p0 = coff[c]
+            n_c = cp.shape[0]
+            P2 = ct.c_void_p
+            mm = np.zeros(1, np.int64)
+            one = np.ascontiguousarray(cp, np.float64)
+            hl.hl_build(one.ctypes.data_as(P2), n_c, 15, S, 0.5, mm.ctypes.data_as(P2), None, None)
+            assert r1 - r0 == mm[0]
+            if n_c == 0:
+                continue
+            t1 = np.zeros((mm[0], 32), np.uint8)
+            perm1 = np.zeros(n_c, np.int32)
+            hl.hl_build(one.ctypes.data_as(P2), n_c, 15, S, 0.5, mm.ctypes.data_as(P2),
+                        t1.ctypes.data_as(P2), perm1.ctypes.data_as(P2))
+            i1 = t1.view(np.int32).reshape(-1, 8)
+            sub = ints[r0:r1]
+            assert np.array_equal(sub[:, 2] - r0, i1[:, 2])          # escape
+            assert np.array_equal(sub[:, 3] - p0, i1[:, 3])          # begin
+            assert np.array_equal(sub[:, 4] - p0, i1[:, 4])          # end
+            assert np.array_equal(sub[:, 5], i1[:, 5])               # depth | leaf
+            par = sub[:, 6].copy()
+            par[par >= 0] -= r0
+            assert np.array_equal(par, i1[:, 6])                     # parent
+            assert np.array_equal(topo[r0:r1].view(np.float64).reshape(-1, 4)[:, 0],
+                                  t1.view(np.float64).reshape(-1, 4)[:, 0])  # thr2
+            assert np.array_equal(perm[p0:p0 + n_c] - p0, perm1)
