@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle sample")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times and exit (calibrates "
+                         "the HBM byte counters FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -69,6 +72,14 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no ROCm GPU visible"}))
         sys.exit(1)
+    if args.copy_probe:
+        src = torch.empty(args.copy_probe << 18, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20}))
+        return
     from cagr_amd import WindingNumberTreecode, synth
 
     dev = torch.device("cuda:0")

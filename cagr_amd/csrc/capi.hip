@@ -251,6 +251,7 @@ int cagr_tree_destroy(cagr_tree* t) {
     (void)hipFree(t->m64);
     (void)hipFree(t->qws);
     (void)hipFree(t->leaf_of);
+    (void)hipFree(t->lbound);
     (void)hipFree(t->items);
     (void)hipFree(t->item_order);
     (void)hipFree(t->item_counter);

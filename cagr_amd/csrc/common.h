@@ -79,6 +79,7 @@ struct cagr_tree {
     uint64_t* keys = nullptr;     // [n] sorted Morton keys
     cagr::TopoRec* topo = nullptr;// [m] preorder
     int32_t* order = nullptr;     // [m] leaves, then internal nodes deepest level first
+    float4* lbound = nullptr;     // [m] bounding sphere {centre, radius} of each leaf's points
     std::vector<int64_t> group_off;  // host: order[] offsets: [0]=leaves, [1+i] internal depth (14-i)
 
     // workspace, owned (sized at build)

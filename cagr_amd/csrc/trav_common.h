@@ -58,6 +58,32 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Deepest node depth at which "far" already proves r >= w for every lane of
+// the wave (far means dist > open_factor * 2^-depth): for those nodes the
+// cutoff test of the far interaction can be skipped.  wmax = the wave's
+// largest width.
+__device__ inline int far_safe_depth(double open_factor, double wmax) {
+    if (!(wmax > 0.0)) return 64;
+    const double ratio = open_factor / (wmax * (1.0 + 1e-4));
+    if (!(ratio >= 1.0)) return -1;
+    return ilogb(ratio);
+}
+
+// Is every point of a leaf (bounding sphere lb) certainly more than w away
+// from the query xf, with margins far beyond FP32 rounding?  Then the leaf
+// needs no cutoff test at all.
+__device__ inline bool leaf_surely_outside(const float* xf, float4 lb, float wf) {
+    const float a = xf[0] - lb.x, b = xf[1] - lb.y, c = xf[2] - lb.z;
+    const float d2 = fmaf(a, a, fmaf(b, b, c * c));
+    const float lim = lb.w + wf * 1.0001f + 1e-6f;
+    return d2 * (1.f - 1e-6f) > lim * lim;
+}
+
 template <typename T>
 __device__ inline T lane_bcast(T v, int lane) {
     if constexpr (sizeof(T) == 4) {

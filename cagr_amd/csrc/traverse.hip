@@ -65,6 +65,8 @@ struct TP {
     int64_t* cnt_far;
     int64_t* cnt_near;
     int64_t* cnt_visit;
+    double open_factor;       // 2 / theta
+    const float4* lbound;     // leaf bounding spheres
 };
 
 using namespace dev;
@@ -206,15 +208,22 @@ __device__ __forceinline__ void trav_v(
     if constexpr (P == CAGR_PREC_FP32C) {
         const NodeV32* __restrict__ nodes = (const NodeV32*)nodes_;
         const SrcV32* __restrict__ src = (const SrcV32*)src_;
-        float xf[3];
+        // query as an unevaluated float sum hi + lo: d = (hi - y) + lo keeps the
+        // FP64 query's position (tree points are exact floats, so lo = 0 there)
+        float xf[3], xl[3] = {0.f, 0.f, 0.f};
         if constexpr (QM == QM_SELF) {
             const int64_t s = valid ? idx : 0;
             xf[0] = pts32[3 * s]; xf[1] = pts32[3 * s + 1]; xf[2] = pts32[3 * s + 2];
         } else {
-            xf[0] = (float)x[0]; xf[1] = (float)x[1]; xf[2] = (float)x[2];
+            for (int k = 0; k < 3; ++k) {
+                xf[k] = (float)x[k];
+                xl[k] = (float)(x[k] - (double)xf[k]);
+            }
         }
         const float wf = (float)w;
         const float w2 = wf * wf;
+        const int dsafe = __builtin_amdgcn_readfirstlane(
+            far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
         VAcc32<VOP, CONT> acc;
         acc.init();
         for (int32_t n = root; n < stop;) {
@@ -233,25 +242,41 @@ __device__ __forceinline__ void trav_v(
                 } else {
                     const float d[3] = {(float)e0, (float)e1, (float)e2};
                     const float r2 = (float)d2;
-                    const bool in = inside_fp32(r2, w2, e0, e1, e2, w);
+                    bool in = false;   // far at depth <= dsafe proves r > w
+                    if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w);
                     acc.add(acc.fp, d, r2, in, nr.m, nr.e, nr.q);
                 }
             }
             const bool nearl = act && !far;
             if (topo_leaf(t)) {
-                if (nearl) {
-                    resume = t.escape;
-                    if constexpr (CNT) {
-                        nnear += t.end - t.begin;
-                    } else {
-                        for (int32_t j = t.begin; j < t.end; ++j) {
-                            const SrcV32 sr = src[j];
-                            const float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
-                            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                            const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
-                                                        x[1] - (double)sr.y[1],
-                                                        x[2] - (double)sr.y[2], w);
-                            acc.add(acc.lp, d, r2, in, sr.mu, sr.e, sr.q);
+                if (nearl) resume = t.escape;
+                if constexpr (CNT) {
+                    if (nearl) nnear += t.end - t.begin;
+                } else if (__ballot(nearl) != 0) {
+                    const bool sure = leaf_surely_outside(xf, p.lbound[n], wf);
+                    const bool fast = __ballot(nearl && !sure) == 0;
+                    if (nearl) {
+                        if (fast) {   // no pair of this leaf is within any lane's width
+                            for (int32_t j = t.begin; j < t.end; ++j) {
+                                const SrcV32 sr = src[j];
+                                float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
+                                if constexpr (QM != QM_SELF)
+                                    for (int k = 0; k < 3; ++k) d[k] += xl[k];
+                                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                                acc.add(acc.lp, d, r2, false, sr.mu, sr.e, sr.q);
+                            }
+                        } else {
+                            for (int32_t j = t.begin; j < t.end; ++j) {
+                                const SrcV32 sr = src[j];
+                                float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
+                                if constexpr (QM != QM_SELF)
+                                    for (int k = 0; k < 3; ++k) d[k] += xl[k];
+                                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                                const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
+                                                            x[1] - (double)sr.y[1],
+                                                            x[2] - (double)sr.y[2], w);
+                                acc.add(acc.lp, d, r2, in, sr.mu, sr.e, sr.q);
+                            }
                         }
                         acc.flush_leaf();
                     }
@@ -385,27 +410,59 @@ __device__ __forceinline__ void trav_s(
     if constexpr (P == CAGR_PREC_FP32C) {
         const NodeS32* __restrict__ nodes = (const NodeS32*)nodes_;
         const SrcS32<K>* __restrict__ src = (const SrcS32<K>*)src_;
-        float xf[3];
+        float xf[3], xl[3] = {0.f, 0.f, 0.f};
         if constexpr (QM == QM_SELF) {
             const int64_t s = valid ? idx : 0;
             xf[0] = pts32[3 * s]; xf[1] = pts32[3 * s + 1]; xf[2] = pts32[3 * s + 2];
         } else {
-            xf[0] = (float)x[0]; xf[1] = (float)x[1]; xf[2] = (float)x[2];
+            for (int k = 0; k < 3; ++k) {
+                xf[k] = (float)x[k];
+                xl[k] = (float)(x[k] - (double)xf[k]);
+            }
         }
         float eq[K];   // exp(An * x_dir / 2) of each set's direction
         for (int k = 0; k < K; ++k) eq[k] = (float)exp(han * sel3(x, p.kdir + k));
         const float hanf = (float)han;
         const float wf = (float)w;
         const float w2 = wf * wf;
+        const int dsafe = __builtin_amdgcn_readfirstlane(
+            far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
         float s[3] = {0.f, 0.f, 0.f}, c[3] = {0.f, 0.f, 0.f}, lp[3] = {0.f, 0.f, 0.f},
               fp[3] = {0.f, 0.f, 0.f};
+        // one near-pair interaction of all K sets (weights mk), into lp
+        auto near_pair = [&](const SrcS32<K>& sr, const float* d, float r2, bool in,
+                             const float* mk) {
+            const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+            const float ir3 = ir * ir * ir;
+            float g[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) g[k] = mk[k] * sr.a[k];
+            float gsum = g[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) gsum += g[k];
+            const float gi3 = gsum * ir3;
+            for (int cc = 0; cc < 3; ++cc) lp[cc] = fmaf(gi3, d[cc], lp[cc]);
+            const float hir = hanf * ir;
+            if constexpr (K == 3) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) lp[k] = fmaf(g[k], hir, lp[k]);
+            } else {
+                const float v = g[0] * hir;
+                for (int cc = 0; cc < 3; ++cc) lp[cc] += (cc == p.kdir) ? v : 0.f;
+            }
+        };
         for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = topo[n];
+            NodeS32 nrs[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) nrs[k] = nodes[(int64_t)n * K + k];
+            const bool safe = topo_depth(t) <= dsafe;
             bool nearl[K];
             bool any_near = false;
+#pragma unroll
             for (int k = 0; k < K; ++k) {
-                const NodeS32 nr = nodes[(int64_t)n * K + k];
+                const NodeS32& nr = nrs[k];
                 const bool act = resume[k] <= n;
                 const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
                 const double d2 = dist2_rn(e0, e1, e2);
@@ -418,48 +475,56 @@ __device__ __forceinline__ void trav_s(
                     } else {
                         const float d[3] = {(float)e0, (float)e1, (float)e2};
                         const float r2 = (float)d2;
-                        const bool in = inside_fp32(r2, w2, e0, e1, e2, w);
+                        bool in = false;   // far at depth <= dsafe proves r > w
+                        if (!safe) in = inside_fp32(r2, w2, e0, e1, e2, w);
                         const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
                         const float g = eq[k] * nr.e * nr.s;
                         const float gi3 = g * (ir * ir * ir);
                         const float hg = hanf * g * ir;
-                        const int dk = p.kdir + k;
-                        for (int cc = 0; cc < 3; ++cc)
-                            fp[cc] = fmaf(gi3, d[cc], fp[cc]) + (cc == dk ? hg : 0.f);
+                        for (int cc = 0; cc < 3; ++cc) fp[cc] = fmaf(gi3, d[cc], fp[cc]);
+                        if constexpr (K == 3) {
+                            fp[k] += hg;
+                        } else {
+                            for (int cc = 0; cc < 3; ++cc) fp[cc] += (cc == p.kdir) ? hg : 0.f;
+                        }
                     }
                 }
                 nearl[k] = act && !far;
                 any_near |= nearl[k];
             }
             if (topo_leaf(t)) {
-                if (any_near) {
-                    for (int k = 0; k < K; ++k)
-                        if (nearl[k]) resume[k] = t.escape;
-                    if constexpr (CNT) {
-                        for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
-                    } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (nearl[k]) resume[k] = t.escape;
+                if constexpr (CNT) {
+                    for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
+                } else if (__ballot(any_near) != 0) {
+                    const bool sure = leaf_surely_outside(xf, p.lbound[n], wf);
+                    const bool fast = __ballot(any_near && !sure) == 0;
+                    if (any_near) {
                         float mk[K];
+#pragma unroll
                         for (int k = 0; k < K; ++k) mk[k] = nearl[k] ? eq[k] : 0.f;
-                        for (int32_t j = t.begin; j < t.end; ++j) {
-                            const SrcS32<K> sr = src[j];
-                            const float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
-                            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                            const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
-                                                        x[1] - (double)sr.y[1],
-                                                        x[2] - (double)sr.y[2], w);
-                            const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
-                            const float ir3 = ir * ir * ir;
-                            float gsum = 0.f;
-                            float g[K];
-                            for (int k = 0; k < K; ++k) { g[k] = mk[k] * sr.a[k]; gsum += g[k]; }
-                            const float gi3 = gsum * ir3;
-                            for (int cc = 0; cc < 3; ++cc) lp[cc] = fmaf(gi3, d[cc], lp[cc]);
-                            const float hir = hanf * ir;
-                            if constexpr (K == 3) {
-                                for (int k = 0; k < 3; ++k) lp[k] = fmaf(g[k], hir, lp[k]);
-                            } else {
-                                const float v = g[0] * hir;
-                                for (int cc = 0; cc < 3; ++cc) lp[cc] += (cc == p.kdir) ? v : 0.f;
+                        if (fast) {   // no pair of this leaf is within any lane's width
+                            for (int32_t j = t.begin; j < t.end; ++j) {
+                                const SrcS32<K> sr = src[j];
+                                float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
+                                if constexpr (QM != QM_SELF)
+                                    for (int k = 0; k < 3; ++k) d[k] += xl[k];
+                                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                                near_pair(sr, d, r2, false, mk);
+                            }
+                        } else {
+                            for (int32_t j = t.begin; j < t.end; ++j) {
+                                const SrcS32<K> sr = src[j];
+                                float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
+                                if constexpr (QM != QM_SELF)
+                                    for (int k = 0; k < 3; ++k) d[k] += xl[k];
+                                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                                const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
+                                                            x[1] - (double)sr.y[1],
+                                                            x[2] - (double)sr.y[2], w);
+                                near_pair(sr, d, r2, in, mk);
                             }
                         }
                         for (int cc = 0; cc < 3; ++cc) {
@@ -756,6 +821,8 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     tp.cnt_far = cnt_far;
     tp.cnt_near = cnt_near;
     tp.cnt_visit = cnt_visit;
+    tp.open_factor = t->open_factor;
+    tp.lbound = t->lbound;
     int64_t lanes;
     if (q.mode == QM_GRID) {
         for (int d = 0; d < 3; ++d) {

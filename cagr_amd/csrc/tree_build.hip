@@ -200,6 +200,41 @@ __global__ __launch_bounds__(kThreads) void k_leaf_of(const TopoRec* __restrict_
         for (int32_t s = t.begin; s < t.end; ++s) leaf_of[s] = (int32_t)i;
 }
 
+// bounding sphere of every leaf's points: centre = middle of their bounding
+// box (rounded to float), radius = the largest FP64 distance to that centre,
+// rounded up; used only to prove that a whole leaf lies outside a width sphere
+__global__ __launch_bounds__(kThreads) void k_leaf_bounds(const TopoRec* __restrict__ topo,
+                                                          int64_t m, const double* __restrict__ p64,
+                                                          float4* __restrict__ lb) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= m) return;
+    const TopoRec t = topo[i];
+    if (!topo_leaf(t)) {
+        lb[i] = make_float4(0.f, 0.f, 0.f, 3.0e38f);
+        return;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int32_t s = t.begin; s < t.end; ++s)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], p64[3 * s + d]);
+            hi[d] = fmax(hi[d], p64[3 * s + d]);
+        }
+    float c[3];
+    for (int d = 0; d < 3; ++d) c[d] = (float)(0.5 * (lo[d] + hi[d]));
+    double r2 = 0.0;
+    for (int32_t s = t.begin; s < t.end; ++s) {
+        double q = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double e = p64[3 * s + d] - (double)c[d];
+            q += e * e;
+        }
+        r2 = fmax(r2, q);
+    }
+    const float r = (float)(sqrt(r2) * (1.0 + 1e-6)) + 1e-7f;
+    // a leaf with a non-finite point never qualifies
+    lb[i] = make_float4(c[0], c[1], c[2], (r2 == r2 && r2 < 1e300) ? r : 3.0e38f);
+}
+
 // group of a node in the upward order: 0 = leaf, 1 + (14 - depth) = internal
 __device__ inline int up_group(const TopoRec& t) {
     return topo_leaf(t) ? 0 : 1 + (kLevels - 1 - topo_depth(t));
@@ -433,6 +468,9 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     CAGR_HIP(hipMalloc(&t->adj_node, 3 * m * sizeof(double)));
     CAGR_HIP(hipMalloc(&t->adj_pt, 3 * n * sizeof(double)));
     k_leaf_of<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->leaf_of);
+    CAGR_LAUNCHED();
+    CAGR_HIP(hipMalloc(&t->lbound, m * sizeof(float4)));
+    k_leaf_bounds<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->pts64, t->lbound);
     CAGR_LAUNCHED();
     CAGR_HIP(hipStreamSynchronize(st));
     return CAGR_OK;
