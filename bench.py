@@ -16,6 +16,12 @@ FP32-compensated form; the plain FP64 operator on the same workload is timed
 beside it (fp64_value), and the oracle (the FP64 CPU restatement) is timed on
 a bounded sample of queries on the host cores (cpu_baseline).
 
+Solve comparison (solve): CGNR on the normal equations of the linear operator
+A mu = -[A21; A22; A23] mu (unit point weights, the same width), b = 0.5,
+to ||A^T(b - A mu)|| / ||A^T b|| <= --solve-tol, once with the plain FP64
+operator and exact adjoint, once with FP32c iterations and FP64-verified
+residual replacement; both start from mu = 0 on the tree built above.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -52,6 +58,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle sample")
+    ap.add_argument("--solve-tol", type=float, default=1e-3,
+                    help="FP64-verified relative residual of the solve comparison")
+    ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times and exit (calibrates "
                          "the HBM byte counters FETCH_SIZE / WRITE_SIZE)")
@@ -162,6 +171,54 @@ def main():
                     fp32c_err_vs_fp64={"A_mu": a_err, "AT_r": r_err})
         log(f"fp64 {fp64['value']:.2f} pairs/s; fp32c normwise err vs fp64: A {a_err:.2e}, AT {r_err:.2e}")
 
+    # ---------------------------------------------------------------- solve
+    solve = None
+    if not args.no_solve:
+        from cagr_amd.solve import NormalOperator, cgnr_solve, cgnr_solve_mixed
+        # warm both precisions' kernels once so compilation and first-touch stay out
+        op = NormalOperator(wt, args.width)
+        for prec in ("fp64", "fp32c"):
+            op.AT(op.A(torch.zeros((n, 3), dtype=torch.float64, device=dev), prec), prec)
+        torch.cuda.synchronize()
+        # per-application times of the solve's operator and its exact adjoint
+        op_ms = {}
+        x3 = mu.double()
+        for prec in ("fp32c", "fp64"):
+            reps = 5
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                y = op.A(x3, prec)
+            torch.cuda.synchronize()
+            ta = (time.perf_counter() - t) / reps * 1e3
+            t = time.perf_counter()
+            for _ in range(reps):
+                op.AT(y, prec)
+            torch.cuda.synchronize()
+            tat = (time.perf_counter() - t) / reps * 1e3
+            op_ms[prec] = {"A": round(ta, 4), "AT": round(tat, 4)}
+        log(f"solve operator ms: {op_ms}")
+        log(f"solve: fp64 CGNR to {args.solve_tol:g}")
+        mu64, i64 = cgnr_solve(wt, args.width, tol=args.solve_tol)
+        log(f"solve: fp64 {i64['iters']} iterations, {i64['seconds']:.3f} s, "
+            f"residual {i64['rel_residual']:.3e}")
+        mu32, i32 = cgnr_solve_mixed(wt, args.width, tol=args.solve_tol)
+        log(f"solve: mixed {i32['iters']} iterations ({i32['restarts']} restarts), "
+            f"{i32['seconds']:.3f} s, residual {i32['rel_residual']:.3e}")
+        dmu = float(torch.linalg.norm(mu32 - mu64) / torch.linalg.norm(mu64))
+        solve = {
+            "tol": args.solve_tol,
+            "fp64": {"iters": i64["iters"], "seconds": round(i64["seconds"], 4),
+                     "rel_residual": i64["rel_residual"]},
+            "mixed": {"iters": i32["iters"], "restarts": i32["restarts"],
+                      "seconds": round(i32["seconds"], 4), "rel_residual": i32["rel_residual"]},
+            "speedup": round(i64["seconds"] / i32["seconds"], 3),
+            "operator_ms": op_ms,
+            "mu_rel_diff": dmu,
+        }
+        log(f"solve: speedup {solve['speedup']:.2f}x, ||mu32 - mu64|| / ||mu64|| = {dmu:.2e}")
+        del mu64, mu32, op
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -207,6 +264,8 @@ def main():
         line["fp64_value"] = round(fp64["value"], 4)
         line["speedup_vs_fp64"] = round(value / fp64["value"], 3)
         line["fp64"] = fp64
+    if solve:
+        line["solve"] = solve
     if cpu:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

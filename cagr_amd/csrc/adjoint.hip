@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             adj_term32<GA, CONT>(z, d, r2, in, h, nr.e, hanf, gin, winv, ew);
         }
         if (__ballot(takefar) != 0) {
-            const float s0 = wave_sum(z[0]), s1 = wave_sum(z[1]), s2 = wave_sum(z[2]);
+            const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]), s2 = wave_sum_dpp(z[2]);
             if (lane == 0) {
                 atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
                 atomicAdd(p.adj_node + 3 * (int64_t)n + 1, (double)s1);
@@ -165,36 +165,54 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             if (nearl) resume = t.escape;
             const uint64_t qmask = __ballot(nearl);
             if (qmask != 0) {
-                for (int32_t base = t.begin; base < t.end; base += 64) {
-                    const int32_t j = base + lane;
+                // Leaves of <= 32 points: both wave halves hold the same 32
+                // sources and take alternate queries of the mask, so every
+                // lane works; the halves' sums meet through one DPP move.
+                const bool split = t.end - t.begin <= 32;
+                const int half = lane >> 5;
+                for (int32_t base = t.begin; base < t.end; base += split ? 32 : 64) {
+                    const int32_t j = base + (split ? (lane & 31) : lane);
                     const bool has = j < t.end;
                     SrcV32 sr;
                     if (has) sr = src[j];
                     float zz[3] = {0.f, 0.f, 0.f};
-                    for (uint64_t mm = qmask; mm != 0; mm &= mm - 1) {
-                        const int qi = __builtin_ctzll(mm);
-                        const float qx[3] = {lane_bcast(xf[0], qi), lane_bcast(xf[1], qi),
-                                             lane_bcast(xf[2], qi)};
-                        const float qh[3] = {lane_bcast(h[0], qi), lane_bcast(h[1], qi),
-                                             lane_bcast(h[2], qi)};
-                        const float qw2 = lane_bcast(w2, qi);
+                    uint64_t mm = qmask;
+                    while (mm != 0) {
+                        int qi = __builtin_ctzll(mm);
+                        mm &= mm - 1;
+                        int qj = qi;       // the upper half's query (split mode)
+                        bool qj_ok = !split;
+                        if (split && mm != 0) {
+                            qj = __builtin_ctzll(mm);
+                            mm &= mm - 1;
+                            qj_ok = true;
+                        }
+                        // the lane's query value (readlane reads inactive lanes too)
+                        auto pick = [&](auto v) {
+                            const auto a = lane_bcast(v, qi);
+                            if (!split) return a;
+                            const auto b = lane_bcast(v, qj);
+                            return half ? b : a;
+                        };
+                        const float qx[3] = {pick(xf[0]), pick(xf[1]), pick(xf[2])};
+                        const float qh[3] = {pick(h[0]), pick(h[1]), pick(h[2])};
+                        const float qw2 = pick(w2);
                         float qg[3] = {0.f, 0.f, 0.f}, qwinv = 0.f, qew = 0.f;
                         if constexpr (CONT) {
-                            qg[0] = lane_bcast(gin[0], qi);
-                            qg[1] = lane_bcast(gin[1], qi);
-                            qg[2] = lane_bcast(gin[2], qi);
-                            qwinv = lane_bcast(winv, qi);
-                            qew = lane_bcast(ew, qi);
+                            qg[0] = pick(gin[0]);
+                            qg[1] = pick(gin[1]);
+                            qg[2] = pick(gin[2]);
+                            qwinv = pick(winv);
+                            qew = pick(ew);
                         }
-                        if (has) {
+                        if (has && (!split || !half || qj_ok)) {
                             const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
                             const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
                             const bool amb = fabsf(r2 - qw2) <= 1e-5f * qw2;
                             bool in = r2 < qw2;
                             if (__builtin_expect(__ballot(amb) != 0, 0)) {
-                                const double qx64[3] = {lane_bcast(x[0], qi), lane_bcast(x[1], qi),
-                                                        lane_bcast(x[2], qi)};
-                                const double qw64 = lane_bcast(w, qi);
+                                const double qx64[3] = {pick(x[0]), pick(x[1]), pick(x[2])};
+                                const double qw64 = pick(w);
                                 if (amb)
                                     in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
                                                          qx64[1] - (double)sr.y[1],
@@ -203,7 +221,10 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                             adj_term32<GA, CONT>(zz, d, r2, in, qh, sr.e, hanf, qg, qwinv, qew);
                         }
                     }
-                    if (has) {
+                    if (split) {   // upper half adds its sums into the lower half
+                        for (int k = 0; k < 3; ++k) zz[k] += __shfl_down(zz[k], 32, 64);
+                    }
+                    if (has && (!split || !half)) {
                         atomicAdd(p.adj_pt + 3 * (int64_t)j, (double)zz[0]);
                         atomicAdd(p.adj_pt + 3 * (int64_t)j + 1, (double)zz[1]);
                         atomicAdd(p.adj_pt + 3 * (int64_t)j + 2, (double)zz[2]);

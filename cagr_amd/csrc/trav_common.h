@@ -58,6 +58,24 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// wave-wide sum through DPP lane moves (no LDS traffic): quad swaps, row
+// mirrors, then the two row broadcasts; the total ends in lane 63 and is
+// returned to every lane.  All 64 lanes must be active.
+template <int CTRL, int ROWMASK>
+__device__ inline float dpp_move(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                                 ROWMASK, 0xf, false));
+}
+__device__ inline float wave_sum_dpp(float v) {
+    v += dpp_move<0xb1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v += dpp_move<0x4e, 0xf>(v);    // quad_perm [2,3,0,1]
+    v += dpp_move<0x141, 0xf>(v);   // row_half_mirror
+    v += dpp_move<0x140, 0xf>(v);   // row_mirror: every lane holds its row's sum
+    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
 __device__ inline double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
