@@ -41,8 +41,10 @@ struct AP {
     const void* widths;
     int wdtype;
     double an, sign;
+    double open_factor;       // 2 / theta
     double* adj_node;         // [m,3] accumulators L
-    double* adj_pt;           // [n,3] accumulators N (sorted order)
+    double* adj_pt;           // [3][n] accumulators N (sorted order, one plane per component,
+                              // so a wave's atomics on one component touch consecutive words)
 };
 
 // cotangent of query oi as the per-direction weights of the k-sum form
@@ -64,21 +66,21 @@ __device__ inline void load_cot(const AP& p, int64_t oi, double gk[3]) {
 
 // ---------------------------------------------------------------- FP32c
 // interaction of (query with factors h, width state) with a source at d:
-// returns the contribution to the source's adjoint accumulator
+// adds the contribution to the source's adjoint accumulator z.  Lanes with
+// use = false add exact zeros, so a wave can run it unmasked.
 template <bool GA, bool CONT>
-__device__ inline void adj_term32(float* z, const float* d, float r2, bool in, const float* h,
-                                  const float* e, float hanf, const float* gin, float winv,
-                                  float ew) {
-    const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+__device__ inline void adj_term32(float* z, const float* d, float r2, bool use, bool in,
+                                  const float* h, const float* e, float hanf, const float* gin,
+                                  float winv, float ew) {
+    const float ir = (use && !in) ? __builtin_amdgcn_rsqf(r2) : 0.f;
     const float ir3 = ir * ir * ir;
+    const bool ci = use && in;
     if constexpr (GA) {
         const float u = h[0] * ir3;
         for (int c = 0; c < 3; ++c) z[c] = fmaf(u, d[c], z[c]);
         if constexpr (CONT) {
-            if (in) {
-                const float v = gin[0] * winv * winv * winv;
-                for (int c = 0; c < 3; ++c) z[c] = fmaf(v, d[c], z[c]);
-            }
+            const float v = ci ? gin[0] * winv * winv * winv : 0.f;
+            for (int c = 0; c < 3; ++c) z[c] = fmaf(v, d[c], z[c]);
         }
     } else {
         const float a0 = h[0] * e[0], a1 = h[1] * e[1], a2 = h[2] * e[2];
@@ -88,12 +90,10 @@ __device__ inline void adj_term32(float* z, const float* d, float r2, bool in, c
         z[1] = fmaf(gi3, d[1], fmaf(a1, hir, z[1]));
         z[2] = fmaf(gi3, d[2], fmaf(a2, hir, z[2]));
         if constexpr (CONT) {
-            if (in) {
-                const float gs = (gin[0] + gin[1]) + gin[2];
-                const float v = ew * gs * winv * winv * winv;
-                const float hw = ew * hanf * winv;
-                for (int c = 0; c < 3; ++c) z[c] = fmaf(v, d[c], fmaf(hw, gin[c], z[c]));
-            }
+            const float gs = (gin[0] + gin[1]) + gin[2];
+            const float v = ci ? ew * gs * winv * winv * winv : 0.f;
+            const float hw = ci ? ew * hanf * winv : 0.f;
+            for (int c = 0; c < 3; ++c) z[c] = fmaf(v, d[c], fmaf(hw, gin[c], z[c]));
         }
     }
 }
@@ -134,25 +134,27 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
     const float ew = CONT ? (float)exp(p.an * w) : 0.f;
     const float hanf = (float)(0.5 * p.an);
     int32_t resume = valid ? p.root : INT_MAX;
+    const int dsafe = __builtin_amdgcn_readfirstlane(
+        far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
 
+    // every branch is wave-uniform; lanes that do not take a step add zeros
     for (int32_t n = p.root; n < p.stop;) {
         n = __builtin_amdgcn_readfirstlane(n);
-        const TopoRec t = topo[n];
-        const NodeV32 nr = nodes[n];
+        const TopoRec t = rec32(topo, n);
+        const NodeV32 nr = rec32(nodes, n);
         const bool act = resume <= n;
         const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
         const double d2 = dist2_rn(e0, e1, e2);
         const bool far = d2 > t.thr2;
         const bool takefar = act && far;
-        float z[3] = {0.f, 0.f, 0.f};
-        if (takefar) {
-            resume = t.escape;
+        if (__ballot(takefar) != 0) {
+            resume = takefar ? t.escape : resume;
             const float d[3] = {(float)e0, (float)e1, (float)e2};
             const float r2 = (float)d2;
-            const bool in = inside_fp32(r2, w2, e0, e1, e2, w);
-            adj_term32<GA, CONT>(z, d, r2, in, h, nr.e, hanf, gin, winv, ew);
-        }
-        if (__ballot(takefar) != 0) {
+            bool in = false;   // far at depth <= dsafe proves r > w
+            if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, takefar);
+            float z[3] = {0.f, 0.f, 0.f};
+            adj_term32<GA, CONT>(z, d, r2, takefar, in, h, nr.e, hanf, gin, winv, ew);
             const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]), s2 = wave_sum_dpp(z[2]);
             if (lane == 0) {
                 atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
@@ -161,24 +163,23 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             }
         }
         const bool nearl = act && !far;
+        const uint64_t qmask = __ballot(nearl);
         if (topo_leaf(t)) {
-            if (nearl) resume = t.escape;
-            const uint64_t qmask = __ballot(nearl);
+            resume = nearl ? t.escape : resume;
             if (qmask != 0) {
                 // Leaves of <= 32 points: both wave halves hold the same 32
                 // sources and take alternate queries of the mask, so every
-                // lane works; the halves' sums meet through one DPP move.
+                // lane works; the halves' sums meet through one lane shift.
                 const bool split = t.end - t.begin <= 32;
                 const int half = lane >> 5;
                 for (int32_t base = t.begin; base < t.end; base += split ? 32 : 64) {
                     const int32_t j = base + (split ? (lane & 31) : lane);
                     const bool has = j < t.end;
-                    SrcV32 sr;
-                    if (has) sr = src[j];
+                    const SrcV32 sr = src[has ? j : t.begin];   // lanes past the leaf read a real record
                     float zz[3] = {0.f, 0.f, 0.f};
                     uint64_t mm = qmask;
                     while (mm != 0) {
-                        int qi = __builtin_ctzll(mm);
+                        const int qi = __builtin_ctzll(mm);
                         mm &= mm - 1;
                         int qj = qi;       // the upper half's query (split mode)
                         bool qj_ok = !split;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                             mm &= mm - 1;
                             qj_ok = true;
                         }
-                        // the lane's query value (readlane reads inactive lanes too)
+                        // the lane's query value (readlane reads any lane)
                         auto pick = [&](auto v) {
                             const auto a = lane_bcast(v, qi);
                             if (!split) return a;
@@ -205,35 +206,34 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                             qwinv = pick(winv);
                             qew = pick(ew);
                         }
-                        if (has && (!split || !half || qj_ok)) {
-                            const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
-                            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                            const bool amb = fabsf(r2 - qw2) <= 1e-5f * qw2;
-                            bool in = r2 < qw2;
-                            if (__builtin_expect(__ballot(amb) != 0, 0)) {
-                                const double qx64[3] = {pick(x[0]), pick(x[1]), pick(x[2])};
-                                const double qw64 = pick(w);
-                                if (amb)
-                                    in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
-                                                         qx64[1] - (double)sr.y[1],
-                                                         qx64[2] - (double)sr.y[2])) >= qw64);
-                            }
-                            adj_term32<GA, CONT>(zz, d, r2, in, qh, sr.e, hanf, qg, qwinv, qew);
+                        const bool use = has && (!split || !half || qj_ok);
+                        const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
+                        const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                        const bool amb = use && fabsf(r2 - qw2) <= 1e-5f * qw2;
+                        bool in = r2 < qw2;
+                        if (__builtin_expect(__ballot(amb) != 0, 0)) {
+                            const double qx64[3] = {pick(x[0]), pick(x[1]), pick(x[2])};
+                            const double qw64 = pick(w);
+                            if (amb)
+                                in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
+                                                     qx64[1] - (double)sr.y[1],
+                                                     qx64[2] - (double)sr.y[2])) >= qw64);
                         }
+                        adj_term32<GA, CONT>(zz, d, r2, use, in, qh, sr.e, hanf, qg, qwinv, qew);
                     }
                     if (split) {   // upper half adds its sums into the lower half
                         for (int k = 0; k < 3; ++k) zz[k] += __shfl_down(zz[k], 32, 64);
                     }
                     if (has && (!split || !half)) {
-                        atomicAdd(p.adj_pt + 3 * (int64_t)j, (double)zz[0]);
-                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 1, (double)zz[1]);
-                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 2, (double)zz[2]);
+                        atomicAdd(p.adj_pt + j, (double)zz[0]);
+                        atomicAdd(p.adj_pt + p.n + j, (double)zz[1]);
+                        atomicAdd(p.adj_pt + 2 * p.n + j, (double)zz[2]);
                     }
                 }
             }
             n = t.escape;
         } else {
-            n = __ballot(nearl) != 0 ? n + 1 : t.escape;
+            n = qmask != 0 ? n + 1 : t.escape;
         }
     }
 }
@@ -295,8 +295,8 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
 
     for (int32_t n = p.root; n < p.stop;) {
         n = __builtin_amdgcn_readfirstlane(n);
-        const TopoRec t = topo[n];
-        const NodeV64 nr = nodes[n];
+        const TopoRec t = rec32(topo, n);
+        const NodeV64 nr = rec32(nodes, n);
         const bool act = resume <= n;
         const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
         const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
@@ -338,9 +338,9 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
                         }
                     }
                     if (has) {
-                        atomicAdd(p.adj_pt + 3 * (int64_t)j, zz[0]);
-                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 1, zz[1]);
-                        atomicAdd(p.adj_pt + 3 * (int64_t)j + 2, zz[2]);
+                        atomicAdd(p.adj_pt + j, zz[0]);
+                        atomicAdd(p.adj_pt + p.n + j, zz[1]);
+                        atomicAdd(p.adj_pt + 2 * p.n + j, zz[2]);
                     }
                 }
             }
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_down(int64_t n, const int32_t*
                                                        TO* __restrict__ out) {
     const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (s >= n) return;
-    double z[3] = {adj_pt[3 * s], adj_pt[3 * s + 1], adj_pt[3 * s + 2]};
+    double z[3] = {adj_pt[s], adj_pt[n + s], adj_pt[2 * n + s]};
     for (int32_t nd = leaf_of[s]; nd >= 0; nd = topo[nd].parent)
         for (int c = 0; c < 3; ++c) z[c] += adj_node[3 * (int64_t)nd + c];
     const int64_t o = perm[s];
@@ -388,6 +388,7 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     p.wdtype = dtype;
     p.an = an;
     p.sign = op == CAGR_OP_A_CPU ? -1.0 : 1.0;
+    p.open_factor = t->open_factor;
     p.adj_node = t->adj_node;
     p.adj_pt = t->adj_pt;
     p.kdir = 0;

@@ -212,9 +212,10 @@ int cagr_set_launch(cagr_tree* t, int persistent, int waves_per_block, int64_t g
 int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
     CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
     if (t->num_items == 0) return CAGR_OK;
-    if (!query_cost) {   // back to natural order
+    if (!query_cost) {   // back to natural order, equal contiguous chunks
         (void)hipFree(t->item_order);
         t->item_order = nullptr;
+        for (int c = 0; c <= kQueues; ++c) t->item_bound[c] = (int32_t)(t->num_items * c / kQueues);
         return CAGR_OK;
     }
     std::vector<int4> items(t->num_items);
@@ -224,13 +225,28 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
     std::vector<int64_t> cost(t->num_items, 0);
     for (int64_t i = 0; i < t->num_items; ++i)
         for (int k = 0; k < items[i].w; ++k) cost[i] += query_cost[perm[items[i].z + k]];
+    // kQueues contiguous chunks of the natural (Morton, cloud by cloud) order
+    // with equal total cost, each ordered longest first
+    int64_t total = 0;
+    for (int64_t i = 0; i < t->num_items; ++i) total += cost[i];
+    int32_t bound[kQueues + 1];
+    bound[0] = 0;
+    int64_t run = 0, i = 0;
+    for (int c = 1; c < kQueues; ++c) {
+        const double target = (double)total * c / kQueues;
+        while (i < t->num_items && (double)(run + cost[i]) <= target) run += cost[i++];
+        bound[c] = (int32_t)i;
+    }
+    bound[kQueues] = (int32_t)t->num_items;
     std::vector<int32_t> order(t->num_items);
-    for (int64_t i = 0; i < t->num_items; ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    for (int64_t j = 0; j < t->num_items; ++j) order[j] = (int32_t)j;
+    for (int c = 0; c < kQueues; ++c)
+        std::stable_sort(order.begin() + bound[c], order.begin() + bound[c + 1],
+                         [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
     if (!t->item_order) CAGR_HIP(hipMalloc(&t->item_order, t->num_items * sizeof(int32_t)));
     CAGR_HIP(hipMemcpy(t->item_order, order.data(), t->num_items * sizeof(int32_t),
                        hipMemcpyHostToDevice));
+    for (int c = 0; c <= kQueues; ++c) t->item_bound[c] = bound[c];
     return CAGR_OK;
 }
 

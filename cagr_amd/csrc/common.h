@@ -57,6 +57,10 @@ struct alignas(16) NodeS64 { double c[3]; double s; };
 struct alignas(16) NodeS32 { double c[3]; float s; float e; };
 //   e = exp(-An*c_{dir}/2) at this set's centre
 
+// persistent traversal queues: one per XCD, counters a cache line apart
+constexpr int kQueues = 8;
+constexpr int kCounterStride = 32;
+
 }  // namespace cagr
 
 // ---------------------------------------------------------------- tree handle
@@ -90,7 +94,7 @@ struct cagr_tree {
     double* m64 = nullptr;        // [3, m, 3]
     int32_t* leaf_of = nullptr;   // [n] leaf id of each sorted point
     double* adj_node = nullptr;   // [m, 3] adjoint far-field accumulators
-    double* adj_pt = nullptr;     // [n, 3] adjoint near-field accumulators
+    double* adj_pt = nullptr;     // [3][n] adjoint near-field accumulators (planar)
     size_t ws_queries = 0;        // grow-only query workspace (explicit-query sorting)
     void* qws = nullptr;
 
@@ -98,10 +102,14 @@ struct cagr_tree {
     // node id ranges [cloud_node[c], cloud_node[c+1]) (host)
     int64_t num_clouds = 1;
     std::vector<int64_t> cloud_pt, cloud_node;
-    // wave work items of the persistent traversal (device): {root, stop, q0, qn}
+    // wave work items of the persistent traversal (device): {root, stop, q0, qn}.
+    // The pull order is split into kQueues chunks, one per XCD (blocks are
+    // dispatched to the XCDs round robin), each with its own counter; a wave
+    // drains its XCD's chunk, then steals from the others.
     int4* items = nullptr;
-    int32_t* item_order = nullptr;   // execution order of the items
-    int32_t* item_counter = nullptr;
+    int32_t* item_order = nullptr;   // pull order of the items (null: natural order)
+    int32_t item_bound[cagr::kQueues + 1] = {};   // chunk c = pull positions [bound[c], bound[c+1])
+    int32_t* item_counter = nullptr; // [kQueues * kCounterStride] chunk counters
     int64_t num_items = 0;
     int persistent = 0;              // 1: traversal through the item queue
     int waves_per_block = 4;         // launch shape of the traversal kernels

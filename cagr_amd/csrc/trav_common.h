@@ -30,16 +30,11 @@ __device__ inline T sel3(const T* a, int i) {
     return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
 }
 
-__device__ inline void neumaier(float& s, float& c, float v) {
-    const float t = s + v;
-    c += (fabsf(s) >= fabsf(v)) ? ((s - t) + v) : ((v - t) + s);
-    s = t;
-}
-
 // r >= w decided as the FP64 reference would: FP32 unless too close to call
+// (lanes with use = false never take the FP64 retest)
 __device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, double xd2,
-                                   double w) {
-    const bool amb = fabsf(r2 - w2) <= 1e-5f * w2;
+                                   double w, bool use = true) {
+    const bool amb = use && fabsf(r2 - w2) <= 1e-5f * w2;
     bool in = r2 < w2;
     if (__builtin_expect(__ballot(amb) != 0, 0)) {
         if (amb) in = !(sqrt(dist2_rn(xd0, xd1, xd2)) >= w);
@@ -100,6 +95,14 @@ __device__ inline bool leaf_surely_outside(const float* xf, float4 lb, float wf)
     const float d2 = fmaf(a, a, fmaf(b, b, c * c));
     const float lim = lb.w + wf * 1.0001f + 1e-6f;
     return d2 * (1.f - 1e-6f) > lim * lim;
+}
+
+// Element i of a record array addressed by a 32-bit byte offset: a scalar
+// load then takes base + SGPR offset (one multiply) instead of 64-bit address
+// arithmetic on every node step.  The host guarantees count * sizeof(T) < 2^32.
+template <typename T>
+__device__ inline const T& rec32(const T* base, int32_t i) {
+    return *(const T*)((const char*)base + (uint64_t)((uint32_t)i * (uint32_t)sizeof(T)));
 }
 
 template <typename T>

@@ -24,9 +24,11 @@
 // Far/near tests run in FP64 in both precisions (the decision must not
 // depend on the precision of the kernel sums); in the FP32 form a cutoff
 // test r >= w that FP32 cannot resolve (|r^2 - w^2| <= 1e-5 w^2) is redone in
-// FP64.  FP32 sums are two-level: plain partial sums per leaf and per run of
-// far interactions, folded into the running total with Neumaier's
-// compensated addition.
+// FP64.  FP32 sums are two-level: plain FP32 partial sums per leaf and per
+// run of far interactions, added into an FP64 running total at every leaf
+// (one conversion and one FP64 add per partial, where an FP32 Neumaier pair
+// costs seven instructions per fold and keeps its old sum live past the new
+// one, which costs register copies on the node steps).
 //
 // The convection factor exp(An*(x-y)_k/2) factors into exp(An*x_k/2) (per
 // query) times exp(-An*y_k/2) (per source, precomputed in the source/node
@@ -117,10 +119,11 @@ __device__ inline bool load_query(const TP& p, const double* __restrict__ pts64,
 template <int VOP, bool CONT>
 struct VAcc32 {
     static constexpr int NO = (VOP == VOP_A) ? 1 : 3;
-    float s[NO], c[NO], lp[NO], fp[NO];
+    double s[NO];         // running sums: FP64 totals of the FP32 partials
+    float lp[NO], fp[NO];
     float P, Q[3];   // continuous-kernel sums inside the width: sum d.mu, sum mu
     __device__ void init() {
-        for (int i = 0; i < NO; ++i) { s[i] = 0.f; c[i] = 0.f; lp[i] = 0.f; fp[i] = 0.f; }
+        for (int i = 0; i < NO; ++i) { s[i] = 0.0; lp[i] = 0.f; fp[i] = 0.f; }
         P = 0.f; Q[0] = Q[1] = Q[2] = 0.f;
     }
     // one interaction: d = x - y (FP32), mu / e / q of the source
@@ -146,10 +149,11 @@ struct VAcc32 {
             }
         }
     }
+    // the leaf partial and the far partial since the last leaf join the FP64
+    // total (the pair sum is exact in FP64, one rounding per fold)
     __device__ void flush_leaf() {
         for (int i = 0; i < NO; ++i) {
-            neumaier(s[i], c[i], lp[i]);
-            neumaier(s[i], c[i], fp[i]);
+            s[i] += (double)lp[i] + (double)fp[i];
             lp[i] = 0.f;
             fp[i] = 0.f;
         }
@@ -228,8 +232,8 @@ __device__ __forceinline__ void trav_v(
         acc.init();
         for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
-            const TopoRec t = topo[n];
-            const NodeV32 nr = nodes[n];
+            const TopoRec t = rec32(topo, n);
+            const NodeV32 nr = rec32(nodes, n);
             const bool act = resume <= n;
             const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
             const double d2 = dist2_rn(e0, e1, e2);
@@ -290,7 +294,7 @@ __device__ __forceinline__ void trav_v(
             acc.flush_leaf();
             if (valid) {
                 double res[3];
-                for (int k = 0; k < acc.NO; ++k) res[k] = (double)acc.s[k] + (double)acc.c[k];
+                for (int k = 0; k < acc.NO; ++k) res[k] = acc.s[k];
                 if constexpr (VOP == VOP_A) {
                     double v = res[0];
                     if constexpr (CONT) v += (double)acc.P / (w * w * w);
@@ -322,8 +326,8 @@ __device__ __forceinline__ void trav_v(
         acc.init();
         for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
-            const TopoRec t = topo[n];
-            const NodeV64 nr = nodes[n];
+            const TopoRec t = rec32(topo, n);
+            const NodeV64 nr = rec32(nodes, n);
             const bool act = resume <= n;
             const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
             const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
@@ -427,8 +431,8 @@ __device__ __forceinline__ void trav_s(
         const float w2 = wf * wf;
         const int dsafe = __builtin_amdgcn_readfirstlane(
             far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
-        float s[3] = {0.f, 0.f, 0.f}, c[3] = {0.f, 0.f, 0.f}, lp[3] = {0.f, 0.f, 0.f},
-              fp[3] = {0.f, 0.f, 0.f};
+        double s[3] = {0.0, 0.0, 0.0};   // FP64 totals of the FP32 partials
+        float lp[3] = {0.f, 0.f, 0.f}, fp[3] = {0.f, 0.f, 0.f};
         // one near-pair interaction of all K sets (weights mk), into lp
         auto near_pair = [&](const SrcS32<K>& sr, const float* d, float r2, bool in,
                              const float* mk) {
@@ -453,10 +457,10 @@ __device__ __forceinline__ void trav_s(
         };
         for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
-            const TopoRec t = topo[n];
+            const TopoRec t = rec32(topo, n);
             NodeS32 nrs[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) nrs[k] = nodes[(int64_t)n * K + k];
+            for (int k = 0; k < K; ++k) nrs[k] = rec32(nodes, n * K + k);
             const bool safe = topo_depth(t) <= dsafe;
             bool nearl[K];
             bool any_near = false;
@@ -528,8 +532,7 @@ __device__ __forceinline__ void trav_s(
                             }
                         }
                         for (int cc = 0; cc < 3; ++cc) {
-                            neumaier(s[cc], c[cc], lp[cc]);
-                            neumaier(s[cc], c[cc], fp[cc]);
+                            s[cc] += (double)lp[cc] + (double)fp[cc];
                             lp[cc] = 0.f;
                             fp[cc] = 0.f;
                         }
@@ -543,8 +546,7 @@ __device__ __forceinline__ void trav_s(
         if constexpr (!CNT) {
             if (valid)
                 for (int cc = 0; cc < 3; ++cc) {
-                    neumaier(s[cc], c[cc], fp[cc]);
-                    store_out(p, 3 * oi + cc, (double)s[cc] + (double)c[cc]);
+                    store_out(p, 3 * oi + cc, s[cc] + (double)fp[cc]);
                 }
         }
     } else {
@@ -565,11 +567,11 @@ __device__ __forceinline__ void trav_s(
         };
         for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
-            const TopoRec t = topo[n];
+            const TopoRec t = rec32(topo, n);
             bool nearl[K];
             bool any_near = false;
             for (int k = 0; k < K; ++k) {
-                const NodeS64 nr = nodes[(int64_t)n * K + k];
+                const NodeS64 nr = rec32(nodes, n * K + k);
                 const bool act = resume[k] <= n;
                 const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
                 const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
@@ -638,23 +640,36 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
 }
 
 // Persistent form: a grid sized to the device pulls wave items {root, stop,
-// q0, qn} (a cloud's tree and up to 64 of its sorted points) from an atomic
-// queue in `order` (longest first when the autotuner has costed them) until
-// the queue is empty.  Any block shape up to 512 threads.
+// q0, qn} (a cloud's tree and up to 64 of its sorted points) from atomic
+// queues until all are empty.  The pull order is cut into one chunk per XCD
+// (block b runs on XCD b mod 8): a chunk is a contiguous Morton range, so an
+// XCD's L2 holds the nodes and sources of its own part of space, and, once
+// the autotuner has costed the items, the chunks carry equal cost and are
+// ordered longest first.  A wave drains its own chunk, then steals from the
+// others.  Any block shape up to 512 threads.
 struct Items {
     const int4* items;
-    const int32_t* order;
-    int32_t count;
-    int32_t* counter;
+    const int32_t* order;        // pull position -> item (null: identity)
+    int32_t bound[kQueues + 1];  // chunk c = pull positions [bound[c], bound[c+1])
+    int32_t* counter;            // chunk c's counter at counter[c * kCounterStride]
 };
 
 __device__ __forceinline__ bool next_item(const Items& q, int4& item) {
-    int it = 0;
-    if ((threadIdx.x & 63) == 0) it = atomicAdd(q.counter, 1);
-    it = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(it, 0));
-    if (it >= q.count) return false;
-    item = q.items[q.order ? q.order[it] : it];
-    return true;
+    const int home = (int)(blockIdx.x % kQueues);
+    for (int s = 0; s < kQueues; ++s) {
+        const int c = (home + s) % kQueues;
+        const int32_t lo = q.bound[c], hi = q.bound[c + 1];
+        if (lo >= hi) continue;
+        int it = 0;
+        if ((threadIdx.x & 63) == 0) it = atomicAdd(q.counter + c * kCounterStride, 1);
+        it = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(it, 0));
+        if (it < hi - lo) {
+            const int32_t pos = lo + it;
+            item = q.items[q.order ? q.order[pos] : pos];
+            return true;
+        }
+    }
+    return false;
 }
 
 template <int P, int VOP, bool CONT, bool CNT>
@@ -679,7 +694,9 @@ __global__ __launch_bounds__(512) void k_trav_s_items(
                                    it.y);
 }
 
-__global__ void k_reset_counter(int32_t* c) { *c = 0; }
+__global__ void k_reset_counter(int32_t* c) {
+    if (threadIdx.x < kQueues) c[threadIdx.x * kCounterStride] = 0;
+}
 
 // ---------------------------------------------------------------- dispatch
 template <int P, int VOP, int QM, bool CNT>
@@ -715,8 +732,9 @@ int launch_s(const TP& tp, cagr_tree* t, int64_t blocks, hipStream_t st) {
 
 template <int P, bool CNT>
 int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t st) {
-    const Items q{t->items, t->item_order, (int32_t)t->num_items, t->item_counter};
-    k_reset_counter<<<1, 1, 0, st>>>(t->item_counter);
+    Items q{t->items, t->item_order, {}, t->item_counter};
+    for (int c = 0; c <= kQueues; ++c) q.bound[c] = t->item_bound[c];
+    k_reset_counter<<<1, 64, 0, st>>>(t->item_counter);
     CAGR_LAUNCHED();
     const int threads = 64 * t->waves_per_block;
     // default: enough blocks for 8 waves per SIMD on every CU

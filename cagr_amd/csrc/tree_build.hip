@@ -392,6 +392,10 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     const int64_t m = m32;
     t->m = m;
     CAGR_REQUIRE(m > 0, CAGR_ERR_STATE, "tree build produced no nodes");
+    // the traversal kernels address node records by 32-bit byte offsets; the
+    // largest record array is 3 scalar-family records (96 bytes) per node
+    CAGR_REQUIRE(m * 96 < (int64_t(1) << 32), CAGR_ERR_ARG,
+                 "tree has more than 44739242 nodes (about 4.9e8 points): too large for one tree");
     if (cloud_off) {
         int32_t* roots_d = nullptr;
         CAGR_HIP(hipMalloc(&roots_d, (nclouds + 1) * sizeof(int32_t)));
@@ -449,7 +453,10 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
             }
         t->num_items = (int64_t)items.size();
         CAGR_HIP(hipMalloc(&t->items, items.size() * sizeof(int4)));
-        CAGR_HIP(hipMalloc(&t->item_counter, sizeof(int32_t)));
+        CAGR_HIP(hipMalloc(&t->item_counter, kQueues * kCounterStride * sizeof(int32_t)));
+        // natural order: equal contiguous chunks (each XCD walks one Morton range)
+        for (int c = 0; c <= kQueues; ++c)
+            t->item_bound[c] = (int32_t)(t->num_items * c / kQueues);
         CAGR_HIP(hipMemcpyAsync(t->items, items.data(), items.size() * sizeof(int4),
                                 hipMemcpyHostToDevice, st));
         CAGR_HIP(hipStreamSynchronize(st));
