@@ -20,7 +20,9 @@ Solve comparison (solve): CGNR on the normal equations of the linear operator
 A mu = -[A21; A22; A23] mu (unit point weights, the same width), b = 0.5,
 to ||A^T(b - A mu)|| / ||A^T b|| <= --solve-tol, once with the plain FP64
 operator and exact adjoint, once with FP32c iterations and FP64-verified
-residual replacement; both start from mu = 0 on the tree built above.
+residual replacement (its adjoint through an adjoint plan, whose build is
+timed inside the solve; also reported without the plan); all start from
+mu = 0 on the tree built above.
 
 Prints one JSON line.
 """
@@ -177,8 +179,14 @@ def main():
         from cagr_amd.solve import NormalOperator, cgnr_solve, cgnr_solve_mixed
         # warm both precisions' kernels once so compilation and first-touch stay out
         op = NormalOperator(wt, args.width)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        opp = NormalOperator(wt, args.width, plan=True)
+        torch.cuda.synchronize()
+        plan_ms = (time.perf_counter() - t) * 1e3
         for prec in ("fp64", "fp32c"):
             op.AT(op.A(torch.zeros((n, 3), dtype=torch.float64, device=dev), prec), prec)
+        opp.AT(op.A(torch.zeros((n, 3), dtype=torch.float64, device=dev), "fp32c"), "fp32c")
         torch.cuda.synchronize()
         # per-application times of the solve's operator and its exact adjoint
         op_ms = {}
@@ -197,6 +205,13 @@ def main():
             torch.cuda.synchronize()
             tat = (time.perf_counter() - t) / reps * 1e3
             op_ms[prec] = {"A": round(ta, 4), "AT": round(tat, 4)}
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            opp.AT(y, "fp32c")
+        torch.cuda.synchronize()
+        op_ms["fp32c"]["AT_planned"] = round((time.perf_counter() - t) / 5 * 1e3, 4)
+        op_ms["plan_build"] = round(plan_ms, 3)
         log(f"solve operator ms: {op_ms}")
         log(f"solve: fp64 CGNR to {args.solve_tol:g}")
         mu64, i64 = cgnr_solve(wt, args.width, tol=args.solve_tol)
@@ -204,7 +219,9 @@ def main():
             f"residual {i64['rel_residual']:.3e}")
         mu32, i32 = cgnr_solve_mixed(wt, args.width, tol=args.solve_tol)
         log(f"solve: mixed {i32['iters']} iterations ({i32['restarts']} restarts), "
-            f"{i32['seconds']:.3f} s, residual {i32['rel_residual']:.3e}")
+            f"{i32['seconds']:.3f} s (adjoint plan built inside), residual {i32['rel_residual']:.3e}")
+        _, ia = cgnr_solve_mixed(wt, args.width, tol=args.solve_tol, plan=False)
+        log(f"solve: mixed without plan {ia['iters']} iterations, {ia['seconds']:.3f} s")
         dmu = float(torch.linalg.norm(mu32 - mu64) / torch.linalg.norm(mu64))
         solve = {
             "tol": args.solve_tol,
@@ -213,11 +230,13 @@ def main():
             "mixed": {"iters": i32["iters"], "restarts": i32["restarts"],
                       "seconds": round(i32["seconds"], 4), "rel_residual": i32["rel_residual"]},
             "speedup": round(i64["seconds"] / i32["seconds"], 3),
+            "mixed_atomic_adjoint": {"iters": ia["iters"], "seconds": round(ia["seconds"], 4),
+                                     "speedup": round(i64["seconds"] / ia["seconds"], 3)},
             "operator_ms": op_ms,
             "mu_rel_diff": dmu,
         }
         log(f"solve: speedup {solve['speedup']:.2f}x, ||mu32 - mu64|| / ||mu64|| = {dmu:.2e}")
-        del mu64, mu32, op
+        del mu64, mu32, op, opp
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None

@@ -32,7 +32,8 @@ EXPORTS = (
     "cagr_tree_export", "cagr_tree_permutation", "cagr_upward", "cagr_apply",
     "cagr_apply_weighted", "cagr_apply_adjoint", "cagr_apply_queries", "cagr_grid_query",
     "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
-    "cagr_order_items", "cagr_profile", "cagr_profile_read",
+    "cagr_order_items", "cagr_profile", "cagr_profile_read", "cagr_adjoint_plan",
+    "cagr_apply_adjoint_planned",
 )
 
 _P, _I, _D, _C = ct.c_void_p, ct.c_int64, ct.c_double, ct.c_int
@@ -57,6 +58,8 @@ _SIGS = {
     "cagr_order_items": (_C, [_P, _P]),
     "cagr_profile": (_C, [_P, _C]),
     "cagr_profile_read": (_C, [_P, _C, _P, _P, _P, _I]),
+    "cagr_adjoint_plan": (_C, [_P, _P, _C, _P]),
+    "cagr_apply_adjoint_planned": (_C, [_P, _C, _C, _P, _C, _P, _C, _D, _P, _P]),
 }
 
 _lib = None

@@ -29,6 +29,10 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 
 enum { GM_A = 0, GM_A2K = 1, GM_A3 = 2, GM_A2 = 3 };
+// what a far step of the FP32 kernel does with the wave's summed contribution
+// to the node: FP64 atomic into L (no plan), or a plain store into the wave's
+// next slot (with a plan); the plan is built by a counting and a filling walk
+enum { FM_ATOMIC = 0, FM_SLOT = 1, FM_COUNT = 2, FM_FILL = 3 };
 
 struct AP {
     int64_t n;
@@ -45,6 +49,14 @@ struct AP {
     double* adj_node;         // [m,3] accumulators L
     double* adj_pt;           // [3][n] accumulators N (sorted order, one plane per component,
                               // so a wave's atomics on one component touch consecutive words)
+    // plan (fixed far-step sequence per wave)
+    const int32_t* wave_off;  // [waves + 1] first slot of each wave
+    const int32_t* slot_pos;  // [slots] node-major position of each slot
+    int32_t* wave_cnt;        // [waves] far steps per wave (counting walk)
+    int32_t* node_cnt;        // [m] far steps per node (counting walk)
+    uint64_t* slot_node;      // [slots] node of each slot (filling walk)
+    int32_t* slot_id;         // [slots] the slot's own index (filling walk)
+    float4* far_val;          // [slots] wave sums of the far steps
 };
 
 // cotangent of query oi as the per-direction weights of the k-sum form
@@ -98,7 +110,7 @@ __device__ inline void adj_term32(float* z, const float* d, float r2, bool use, 
     }
 }
 
-template <int GMODE_A, bool CONT>
+template <int GMODE_A, bool CONT, int FM>
 __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restrict__ topo,
                                                     const NodeV32* __restrict__ nodes,
                                                     const SrcV32* __restrict__ src,
@@ -113,10 +125,11 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
     const int64_t sc = valid ? s : 0;
     const double x[3] = {pts64[3 * sc], pts64[3 * sc + 1], pts64[3 * sc + 2]};
     const float xf[3] = {pts32[3 * sc], pts32[3 * sc + 1], pts32[3 * sc + 2]};
+    constexpr bool MATH = FM == FM_ATOMIC || FM == FM_SLOT;   // the walks of the plan do no math
     const int64_t oi = p.perm[sc];
-    const double w = load_as_double(p.widths, p.wdtype, oi);
-    double gk[3];
-    load_cot(p, oi, gk);
+    const double w = MATH ? load_as_double(p.widths, p.wdtype, oi) : 1.0;
+    double gk[3] = {0.0, 0.0, 0.0};
+    if constexpr (MATH) load_cot(p, oi, gk);
     if (!valid) gk[0] = gk[1] = gk[2] = 0.0;
     float h[3], gin[3];
     if constexpr (GA) {
@@ -136,6 +149,14 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
     int32_t resume = valid ? p.root : INT_MAX;
     const int dsafe = __builtin_amdgcn_readfirstlane(
         far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
+    // the wave's slots (plan): [slot0, slot0 + kmax)
+    int64_t slot0 = 0;
+    int32_t kmax = 0;
+    if constexpr (FM == FM_SLOT || FM == FM_FILL) {
+        slot0 = p.wave_off[wave];
+        kmax = p.wave_off[wave + 1] - (int32_t)slot0;
+    }
+    int32_t k = 0;   // far steps so far
 
     // every branch is wave-uniform; lanes that do not take a step add zeros
     for (int32_t n = p.root; n < p.stop;) {
@@ -149,24 +170,38 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         const bool takefar = act && far;
         if (__ballot(takefar) != 0) {
             resume = takefar ? t.escape : resume;
-            const float d[3] = {(float)e0, (float)e1, (float)e2};
-            const float r2 = (float)d2;
-            bool in = false;   // far at depth <= dsafe proves r > w
-            if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, takefar);
-            float z[3] = {0.f, 0.f, 0.f};
-            adj_term32<GA, CONT>(z, d, r2, takefar, in, h, nr.e, hanf, gin, winv, ew);
-            const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]), s2 = wave_sum_dpp(z[2]);
-            if (lane == 0) {
-                atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
-                atomicAdd(p.adj_node + 3 * (int64_t)n + 1, (double)s1);
-                atomicAdd(p.adj_node + 3 * (int64_t)n + 2, (double)s2);
+            if constexpr (FM == FM_COUNT) {
+                if (lane == 0) atomicAdd(p.node_cnt + n, 1);
+            } else if constexpr (FM == FM_FILL) {
+                if (lane == 0 && k < kmax) {
+                    p.slot_node[slot0 + k] = (uint64_t)n;
+                    p.slot_id[slot0 + k] = (int32_t)(slot0 + k);
+                }
+            } else {
+                const float d[3] = {(float)e0, (float)e1, (float)e2};
+                const float r2 = (float)d2;
+                bool in = false;   // far at depth <= dsafe proves r > w
+                if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, takefar);
+                float z[3] = {0.f, 0.f, 0.f};
+                adj_term32<GA, CONT>(z, d, r2, takefar, in, h, nr.e, hanf, gin, winv, ew);
+                const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]),
+                            s2 = wave_sum_dpp(z[2]);
+                if constexpr (FM == FM_SLOT) {
+                    if (lane == 0 && k < kmax)
+                        p.far_val[p.slot_pos[slot0 + k]] = make_float4(s0, s1, s2, 0.f);
+                } else if (lane == 0) {
+                    atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
+                    atomicAdd(p.adj_node + 3 * (int64_t)n + 1, (double)s1);
+                    atomicAdd(p.adj_node + 3 * (int64_t)n + 2, (double)s2);
+                }
             }
+            ++k;
         }
         const bool nearl = act && !far;
         const uint64_t qmask = __ballot(nearl);
         if (topo_leaf(t)) {
             resume = nearl ? t.escape : resume;
-            if (qmask != 0) {
+            if (MATH && qmask != 0) {
                 // Leaves of <= 32 points: both wave halves hold the same 32
                 // sources and take alternate queries of the mask, so every
                 // lane works; the halves' sums meet through one lane shift.
@@ -188,13 +223,10 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                             mm &= mm - 1;
                             qj_ok = true;
                         }
-                        // the lane's query value (readlane reads any lane)
-                        auto pick = [&](auto v) {
-                            const auto a = lane_bcast(v, qi);
-                            if (!split) return a;
-                            const auto b = lane_bcast(v, qj);
-                            return half ? b : a;
-                        };
+                        // the lane's query value, fetched through the LDS
+                        // crossbar (ds_bpermute; every lane is active here)
+                        const int qsrc = (split && half) ? qj : qi;
+                        auto pick = [&](auto v) { return __shfl(v, qsrc, 64); };
                         const float qx[3] = {pick(xf[0]), pick(xf[1]), pick(xf[2])};
                         const float qh[3] = {pick(h[0]), pick(h[1]), pick(h[2])};
                         const float qw2 = pick(w2);
@@ -236,6 +268,52 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             n = qmask != 0 ? n + 1 : t.escape;
         }
     }
+    if constexpr (FM == FM_COUNT)
+        if (lane == 0) p.wave_cnt[wave] = k;
+}
+
+// L of every node from its run of slots, one wave per node: lanes stride the
+// run (coalesced), then a fixed-order wave reduction in FP64, so the planned
+// adjoint is deterministic
+__global__ __launch_bounds__(kThreads) void k_far_gather(int64_t m, const int32_t* __restrict__ node_off,
+                                                         const float4* __restrict__ far_val,
+                                                         double* __restrict__ adj_node) {
+    const int64_t nd = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (nd >= m) return;   // whole waves
+    const int lane = threadIdx.x & 63;
+    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    const int32_t hi = node_off[nd + 1];
+    for (int32_t i = node_off[nd] + lane; i < hi; i += 64) {
+        const float4 v = far_val[i];
+        z0 += (double)v.x;
+        z1 += (double)v.y;
+        z2 += (double)v.z;
+    }
+    z0 = wave_sum(z0);
+    z1 = wave_sum(z1);
+    z2 = wave_sum(z2);
+    if (lane == 0) {
+        adj_node[3 * nd] = z0;
+        adj_node[3 * nd + 1] = z1;
+        adj_node[3 * nd + 2] = z2;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_invert(const int32_t* __restrict__ perm, int64_t len,
+                                                     int32_t* __restrict__ pos) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < len) pos[perm[i]] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(kThreads) void k_to_f64(const void* __restrict__ a, int dtype,
+                                                     int64_t len, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < len) out[i] = load_as_double(a, dtype, i);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill0_i32(int32_t* __restrict__ a, int64_t len) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < len) a[i] = 0;
 }
 
 // ---------------------------------------------------------------- FP64
@@ -376,7 +454,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_down(int64_t n, const int32_t*
 }  // namespace
 
 int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
-            int continuous, double an, void* out, hipStream_t st) {
+            int continuous, double an, void* out, hipStream_t st, bool planned) {
     AP p{};
     p.n = t->n;
     p.root = 0;
@@ -398,21 +476,39 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     else { p.gmode = GM_A2K; p.kdir = op - CAGR_OP_A21; }
     const bool ga = p.gmode == GM_A;
     const int64_t n = t->n, m = t->m;
-    k_fill0<<<(3 * m + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_node, 3 * m);
-    CAGR_LAUNCHED();
+    const bool slots = planned && precision == CAGR_PREC_FP32C;
+    CAGR_REQUIRE(!planned || t->plan_slots >= 0, CAGR_ERR_STATE, "no adjoint plan on this tree");
+    if (slots) {
+        p.wave_off = t->plan_wave_off;
+        p.slot_pos = t->plan_slot_pos;
+        p.far_val = t->plan_far_val;
+    } else {
+        k_fill0<<<(3 * m + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_node, 3 * m);
+        CAGR_LAUNCHED();
+    }
     k_fill0<<<(3 * n + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_pt, 3 * n);
     CAGR_LAUNCHED();
     const int64_t blocks = ((n + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock;
     if (precision == CAGR_PREC_FP32C) {
         auto nodes = (const NodeV32*)t->nodes;
         auto src = (const SrcV32*)t->src;
+#define CAGR_ADJ32(GMA, CT)                                                                      \
+    do {                                                                                         \
+        if (slots)                                                                               \
+            k_adj32<GMA, CT, FM_SLOT><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src,       \
+                                                                   t->pts64, t->pts32);          \
+        else                                                                                     \
+            k_adj32<GMA, CT, FM_ATOMIC><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src,     \
+                                                                     t->pts64, t->pts32);        \
+    } while (0)
         if (ga) {
-            if (continuous) k_adj32<1, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
-            else k_adj32<1, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+            if (continuous) CAGR_ADJ32(1, true);
+            else CAGR_ADJ32(1, false);
         } else {
-            if (continuous) k_adj32<0, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
-            else k_adj32<0, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+            if (continuous) CAGR_ADJ32(0, true);
+            else CAGR_ADJ32(0, false);
         }
+#undef CAGR_ADJ32
     } else {
         auto nodes = (const NodeV64*)t->nodes;
         auto src = (const SrcV64*)t->src;
@@ -425,12 +521,107 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
         }
     }
     CAGR_LAUNCHED();
+    if (slots) {
+        k_far_gather<<<(m + kWavesPerBlock - 1) / kWavesPerBlock, kThreads, 0, st>>>(
+            m, t->plan_node_off, t->plan_far_val, t->adj_node);
+        CAGR_LAUNCHED();
+    }
     const int64_t nb = (n + kThreads - 1) / kThreads;
     if (dtype == CAGR_F32)
         k_adj_down<float><<<nb, kThreads, 0, st>>>(n, t->perm, t->leaf_of, t->topo, t->adj_node, t->adj_pt, (float*)out);
     else
         k_adj_down<double><<<nb, kThreads, 0, st>>>(n, t->perm, t->leaf_of, t->topo, t->adj_node, t->adj_pt, (double*)out);
     CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+
+void adjoint_plan_free(cagr_tree* t) {
+    (void)hipFree(t->plan_w);
+    (void)hipFree(t->plan_wave_off);
+    (void)hipFree(t->plan_node_off);
+    (void)hipFree(t->plan_slot_pos);
+    (void)hipFree(t->plan_far_val);
+    t->plan_w = nullptr;
+    t->plan_wave_off = t->plan_node_off = t->plan_slot_pos = nullptr;
+    t->plan_far_val = nullptr;
+    t->plan_slots = -1;
+}
+
+int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
+    adjoint_plan_free(t);
+    const int64_t n = t->n, m = t->m;
+    CAGR_HIP(hipMalloc(&t->plan_w, n * sizeof(double)));
+    k_to_f64<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>(weights, dtype, n, t->plan_w);
+    CAGR_LAUNCHED();
+    // centres of these weights (the walk needs no moments)
+    int rc = upward(t, 0, CAGR_PREC_FP32C, 1, 0, nullptr, CAGR_F64, t->plan_w, 0.0, st);
+    if (rc) return rc;
+    const int64_t blocks = ((n + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t nw = blocks * kWavesPerBlock;
+    CAGR_HIP(hipMalloc(&t->plan_wave_off, (nw + 1) * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&t->plan_node_off, (m + 1) * sizeof(int32_t)));
+    k_fill0_i32<<<(nw + 1 + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->plan_wave_off, nw + 1);
+    CAGR_LAUNCHED();
+    k_fill0_i32<<<(m + 1 + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->plan_node_off, m + 1);
+    CAGR_LAUNCHED();
+    AP p{};
+    p.n = n;
+    p.root = 0;
+    p.stop = (int32_t)m;
+    p.perm = t->perm;
+    p.open_factor = t->open_factor;
+    p.wave_cnt = t->plan_wave_off;
+    p.node_cnt = t->plan_node_off;
+    auto nodes = (const NodeV32*)t->nodes;
+    auto src = (const SrcV32*)t->src;
+    k_adj32<0, false, FM_COUNT><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+    CAGR_LAUNCHED();
+    // counts -> offsets
+    int32_t* tmp = nullptr;
+    CAGR_HIP(hipMalloc(&tmp, scan_tmp_len(std::max(nw, m) + 1) * sizeof(int32_t)));
+    struct FreeTmp { int32_t** q; ~FreeTmp() { (void)hipFree(*q); } } free_tmp{&tmp};
+    if ((rc = scan_exclusive<int32_t>(t->plan_wave_off, nw + 1, tmp, st))) return rc;
+    if ((rc = scan_exclusive<int32_t>(t->plan_node_off, m + 1, tmp, st))) return rc;
+    int32_t slots32 = 0;
+    CAGR_HIP(hipMemcpyAsync(&slots32, t->plan_wave_off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    const int64_t F = slots32;
+    CAGR_REQUIRE(F >= 0, CAGR_ERR_ARG, "adjoint plan: more than 2^31 far steps");
+    CAGR_HIP(hipMalloc(&t->plan_far_val, std::max<int64_t>(F, 1) * sizeof(float4)));
+    CAGR_HIP(hipMalloc(&t->plan_slot_pos, std::max<int64_t>(F, 1) * sizeof(int32_t)));
+    if (F > 0) {
+        // the node of every slot; the slots grouped by node (stable: by slot
+        // id); the inverse of that order is each slot's node-major position
+        uint64_t *keys = nullptr, *ktmp = nullptr;
+        int32_t *ids = nullptr, *vtmp = nullptr;
+        uint32_t* hist = nullptr;
+        struct FreeAll {
+            uint64_t **a, **b;
+            int32_t **c, **e;
+            uint32_t** d;
+            ~FreeAll() {
+                (void)hipFree(*a); (void)hipFree(*b); (void)hipFree(*c); (void)hipFree(*d);
+                (void)hipFree(*e);
+            }
+        } free_all{&keys, &ktmp, &vtmp, &ids, &hist};
+        CAGR_HIP(hipMalloc(&keys, F * sizeof(uint64_t)));
+        CAGR_HIP(hipMalloc(&ktmp, F * sizeof(uint64_t)));
+        CAGR_HIP(hipMalloc(&ids, F * sizeof(int32_t)));
+        CAGR_HIP(hipMalloc(&vtmp, F * sizeof(int32_t)));
+        CAGR_HIP(hipMalloc(&hist, sort_hist_len(F) * sizeof(uint32_t)));
+        p.wave_off = t->plan_wave_off;
+        p.slot_node = keys;
+        p.slot_id = ids;
+        k_adj32<0, false, FM_FILL><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+        CAGR_LAUNCHED();
+        int bits = 1;
+        while ((int64_t(1) << bits) < m) ++bits;
+        if ((rc = sort_pairs_u64(keys, ids, ktmp, vtmp, hist, F, bits, st))) return rc;
+        k_invert<<<(F + kThreads - 1) / kThreads, kThreads, 0, st>>>(ids, F, t->plan_slot_pos);
+        CAGR_LAUNCHED();
+        CAGR_HIP(hipStreamSynchronize(st));
+    }
+    t->plan_slots = F;
     return CAGR_OK;
 }
 

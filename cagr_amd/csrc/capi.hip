@@ -252,6 +252,7 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
 
 int cagr_tree_destroy(cagr_tree* t) {
     if (!t) return CAGR_OK;
+    adjoint_plan_free(t);
     for (auto& ev : t->prof_events)
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     (void)hipFree(t->pts64);
@@ -375,6 +376,40 @@ int cagr_apply_adjoint(cagr_tree* t, int op, int precision, const void* g, int d
     if ((rc = prof_mark(t, 1, st))) return rc;
     if ((rc = prof_mark(t, 0, st))) return rc;
     rc = adjoint(t, op, precision, g, dtype, widths, continuous, an, out, st);
+    if (rc) return rc;
+    return prof_mark(t, 0, st);
+}
+
+int cagr_adjoint_plan(cagr_tree* t, const void* weights, int dtype, void* stream) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    CAGR_REQUIRE(dtype == CAGR_F32 || dtype == CAGR_F64, CAGR_ERR_ARG, "bad dtype");
+    CAGR_REQUIRE(t->num_clouds == 1, CAGR_ERR_ARG, "the adjoint plan needs a single-cloud tree");
+    if (t->n == 0) return CAGR_OK;
+    CAGR_REQUIRE(weights, CAGR_ERR_ARG, "null weights");
+    const int rc = adjoint_plan(t, weights, dtype, (hipStream_t)stream);
+    if (rc) adjoint_plan_free(t);
+    return rc;
+}
+
+int cagr_apply_adjoint_planned(cagr_tree* t, int op, int precision, const void* g, int dtype,
+                               const void* widths, int continuous, double an, void* out,
+                               void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, precision, dtype, &oi);
+    if (rc) return rc;
+    CAGR_REQUIRE(op == CAGR_OP_A || op == CAGR_OP_A_CPU || op == CAGR_OP_A3 || op == CAGR_OP_A2 ||
+                     (op >= CAGR_OP_A21 && op <= CAGR_OP_A23),
+                 CAGR_ERR_ARG, "adjoint supports A, A_CPU, A21..A23, A2 and A3");
+    if (t->n == 0) return CAGR_OK;
+    CAGR_REQUIRE(t->plan_slots >= 0, CAGR_ERR_STATE, "no adjoint plan: call cagr_adjoint_plan");
+    CAGR_REQUIRE(g && widths && out, CAGR_ERR_ARG, "null g / widths / out");
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = prof_mark(t, 1, st))) return rc;
+    rc = upward(t, 0, precision, 1, 0, nullptr, CAGR_F64, t->plan_w, oi.convects ? an : 0.0, st);
+    if (rc) return rc;
+    if ((rc = prof_mark(t, 1, st))) return rc;
+    if ((rc = prof_mark(t, 0, st))) return rc;
+    rc = adjoint(t, op, precision, g, dtype, widths, continuous, an, out, st, true);
     if (rc) return rc;
     return prof_mark(t, 0, st);
 }

@@ -111,6 +111,16 @@ struct cagr_tree {
     int32_t item_bound[cagr::kQueues + 1] = {};   // chunk c = pull positions [bound[c], bound[c+1])
     int32_t* item_counter = nullptr; // [kQueues * kCounterStride] chunk counters
     int64_t num_items = 0;
+    // adjoint plan: the fixed far-step sequence of the FP32 adjoint for the
+    // node centres of plan_w; far step k of wave w is slot wave_off[w] + k,
+    // stored at slot_pos[slot] in node-major order, so L of a node is the sum
+    // of the contiguous run [node_off[node], node_off[node + 1]) of far_val
+    double* plan_w = nullptr;          // [n] the plan's point weights (original order)
+    int32_t* plan_wave_off = nullptr;  // [waves + 1]
+    int32_t* plan_node_off = nullptr;  // [m + 1]
+    int32_t* plan_slot_pos = nullptr;  // [slots] node-major position of each slot
+    float4* plan_far_val = nullptr;    // [slots] node-major
+    int64_t plan_slots = -1;           // -1: no plan
     int persistent = 0;              // 1: traversal through the item queue
     int waves_per_block = 4;         // launch shape of the traversal kernels
     int num_cus = 256;               // compute units of the device
@@ -135,6 +145,10 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
 int sort_pairs_u64(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp,
                    uint32_t* hist, int64_t n, int bits, hipStream_t st);
 int64_t sort_hist_len(int64_t n);
+// exclusive prefix sum of a[0..len) in place; tmp holds scan_tmp_len(len) elements
+int64_t scan_tmp_len(int64_t len);
+template <typename U>
+int scan_exclusive(U* a, int64_t len, U* tmp, hipStream_t st);
 // implemented in upward.hip
 int upward(cagr_tree* t, int family, int precision, int K, int dir0, const void* attrs,
            int dtype, const void* weights, double an, hipStream_t st);
@@ -153,8 +167,13 @@ struct Queries {
 };
 // implemented in adjoint.hip (needs the node records of an upward pass with
 // the same fixed weights)
+// (planned: FP32 far steps go through the tree's adjoint plan instead of atomics)
 int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
-            int continuous, double an, void* out, hipStream_t st);
+            int continuous, double an, void* out, hipStream_t st, bool planned = false);
+// builds the adjoint plan for point weights (original order; centres of an
+// upward pass with them)
+int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st);
+void adjoint_plan_free(cagr_tree* t);
 int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuous, double an,
              void* out, int odtype, int64_t* cnt_far, int64_t* cnt_near, int64_t* cnt_visit,
              hipStream_t st);

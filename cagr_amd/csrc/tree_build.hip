@@ -74,29 +74,82 @@ __global__ __launch_bounds__(kThreads) void k_rs_hist(const uint64_t* __restrict
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of a[0..len) in one block of 1024 threads
+// ---------------------------------------------------------------- scan
+// Exclusive prefix sum over a device array: tile sums (one block per 4096
+// elements, coalesced loads), a scan of the tile sums (recursively), then a
+// scan of each tile offset by its tile's prefix.  Tiles pass through LDS so
+// every global access is coalesced.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+constexpr __host__ __device__ int scan_pos(int i) { return i + (i >> 5); }   // skew: no LDS bank clashes
+
+// exclusive scan of one value per thread over the 256-thread block; *total
+// receives the block total
 template <typename U>
-__global__ __launch_bounds__(1024) void k_scan_excl(U* __restrict__ a, int64_t len) {
-    __shared__ U part[1024];
-    const int t = threadIdx.x;
-    const int64_t chunk = (len + 1023) / 1024;
-    const int64_t lo = t * chunk;
-    const int64_t hi = lo + chunk < len ? lo + chunk : len;
-    U s = 0;
-    for (int64_t i = lo; i < hi; ++i) s += a[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const U v = t >= off ? part[t - off] : (U)0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+__device__ inline U block_scan_excl(U v, U* wsum, U& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    U x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const U y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    U run = part[t] - s;
-    for (int64_t i = lo; i < hi; ++i) {
-        const U v = a[i];
-        a[i] = run;
-        run += v;
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    U pre = 0;
+    total = 0;
+    for (int k = 0; k < kScanThreads / 64; ++k) {
+        if (k < w) pre += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    return pre + x - v;
+}
+
+template <typename U>
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(const U* __restrict__ a, int64_t len,
+                                                            U* __restrict__ sums) {
+    __shared__ U wsum[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    U s = 0;
+    for (int r = 0; r < kScanPer; ++r) {
+        const int64_t i = base + r * kScanThreads + threadIdx.x;
+        if (i < len) s += a[i];
+    }
+    U total;
+    (void)block_scan_excl(s, wsum, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+template <typename U>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(U* __restrict__ a, int64_t len,
+                                                             const U* __restrict__ offs) {
+    __shared__ U tile[scan_pos(kScanTile)];
+    __shared__ U wsum[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    const int t = threadIdx.x;
+    for (int r = 0; r < kScanPer; ++r) {
+        const int j = r * kScanThreads + t;
+        tile[scan_pos(j)] = base + j < len ? a[base + j] : (U)0;
+    }
+    __syncthreads();
+    U v[kScanPer];
+    U s = 0;
+    for (int r = 0; r < kScanPer; ++r) {
+        v[r] = tile[scan_pos(t * kScanPer + r)];
+        s += v[r];
+    }
+    U total;
+    U run = block_scan_excl(s, wsum, total) + (offs ? offs[blockIdx.x] : (U)0);
+    for (int r = 0; r < kScanPer; ++r) {
+        tile[scan_pos(t * kScanPer + r)] = run;
+        run += v[r];
+    }
+    __syncthreads();
+    for (int r = 0; r < kScanPer; ++r) {
+        const int j = r * kScanThreads + t;
+        if (base + j < len) a[base + j] = tile[scan_pos(j)];
     }
 }
 
@@ -260,7 +313,39 @@ __global__ __launch_bounds__(kThreads) void k_group_fill(const TopoRec* __restri
 
 }  // namespace
 
-int64_t sort_hist_len(int64_t n) { return 256 * std::max<int64_t>(1, (n + kRsTile - 1) / kRsTile); }
+int64_t scan_tmp_len(int64_t len) {
+    int64_t total = 0;
+    while (len > kScanTile) {
+        len = (len + kScanTile - 1) / kScanTile;
+        total += len;
+    }
+    return total + 1;
+}
+
+template <typename U>
+int scan_exclusive(U* a, int64_t len, U* tmp, hipStream_t st) {
+    if (len <= 0) return CAGR_OK;
+    if (len <= kScanTile) {
+        k_scan_tiles<U><<<1, kScanThreads, 0, st>>>(a, len, nullptr);
+        CAGR_LAUNCHED();
+        return CAGR_OK;
+    }
+    const int64_t nt = (len + kScanTile - 1) / kScanTile;
+    k_scan_sums<U><<<nt, kScanThreads, 0, st>>>(a, len, tmp);
+    CAGR_LAUNCHED();
+    int rc = scan_exclusive<U>(tmp, nt, tmp + nt, st);
+    if (rc) return rc;
+    k_scan_tiles<U><<<nt, kScanThreads, 0, st>>>(a, len, tmp);
+    CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+template int scan_exclusive<int32_t>(int32_t*, int64_t, int32_t*, hipStream_t);
+template int scan_exclusive<uint32_t>(uint32_t*, int64_t, uint32_t*, hipStream_t);
+
+int64_t sort_hist_len(int64_t n) {
+    const int64_t h = 256 * std::max<int64_t>(1, (n + kRsTile - 1) / kRsTile);
+    return h + scan_tmp_len(h);
+}
 
 // Sorts (keys, vals) by the low `bits` bits of keys, stably.  The result is
 // left in keys/vals; *_tmp and hist (sort_hist_len(n) entries) are scratch.
@@ -276,8 +361,8 @@ int sort_pairs_u64(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* v
     for (int shift = 0; shift < bits; shift += 8, ++passes) {
         k_rs_hist<<<nb, kThreads, 0, st>>>(ka, n, shift, hist, nb);
         CAGR_LAUNCHED();
-        k_scan_excl<uint32_t><<<1, 1024, 0, st>>>(hist, 256 * nb);
-        CAGR_LAUNCHED();
+        const int rcs = scan_exclusive<uint32_t>(hist, 256 * nb, hist + 256 * nb, st);
+        if (rcs) return rcs;
         k_rs_scatter<<<nb, kThreads, 0, st>>>(ka, va, kb, vb, n, shift, hist, nb);
         CAGR_LAUNCHED();
         std::swap(ka, kb);
@@ -384,8 +469,9 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
 
     k_chain<<<blocks_for(n + 1), kThreads, 0, st>>>(t->keys, n, leaf_size, max_depth, ld, sd, cnt);
     CAGR_LAUNCHED();
-    k_scan_excl<int32_t><<<1, 1024, 0, st>>>(cnt, n + 1);
-    CAGR_LAUNCHED();
+    // the radix histogram buffer is free again and long enough for the scan's tile sums
+    rc = scan_exclusive<int32_t>(cnt, n + 1, (int32_t*)hist, st);
+    if (rc) return rc;
     int32_t m32 = 0;
     CAGR_HIP(hipMemcpyAsync(&m32, cnt + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));

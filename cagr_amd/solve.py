@@ -82,14 +82,20 @@ def cagr_schedule(wt: WindingNumberTreecode, wsmin: float, wsmax: float, iters: 
 
 class NormalOperator:
     """A mu = -[A21 mu; A22 mu; A23 mu] (as [N,3]) with node centres fixed by
-    unit weights, and its exact adjoint; counts operator applications."""
+    unit weights, and its exact adjoint; counts operator applications.  With
+    ``plan`` the FP32c adjoint runs through the tree's adjoint plan (built
+    here, once: the centres never change during a solve)."""
 
-    def __init__(self, wt: WindingNumberTreecode, width: float, an: float = 1.0):
+    def __init__(self, wt: WindingNumberTreecode, width: float, an: float = 1.0,
+                 plan: bool = False):
         n = wt.points.shape[0]
         self.wt, self.an = wt, an
         self.w64 = torch.full((n,), width, dtype=torch.float64, device=wt.device)
         self.ones64 = torch.ones(n, dtype=torch.float64, device=wt.device)
         self.calls = {"fp64": 0, "fp32c": 0}
+        self.plan = plan
+        if plan:
+            wt.adjoint_plan(self.ones64)
 
     def A(self, mu, precision):
         self.calls[precision] += 1
@@ -98,6 +104,9 @@ class NormalOperator:
 
     def AT(self, g, precision):
         self.calls[precision] += 1
+        if self.plan and precision == "fp32c":
+            return -self.wt.apply_adjoint_planned("A2", g, self.w64, an=self.an,
+                                                  precision=precision)
         return -self.wt.apply_adjoint("A2", g, self.w64, self.ones64, an=self.an,
                                       precision=precision)
 
@@ -131,10 +140,11 @@ def _cg(op: NormalOperator, rhs, x0, tol_abs, max_iter, precision):
 def cgnr_solve(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
                max_iter: int = 5000, an: float = 1.0):
     """Plain FP64 CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     op = NormalOperator(wt, width, an)
     n = wt.points.shape[0]
     bt = torch.full((n, 3), b, dtype=torch.float64, device=wt.device)
-    t0 = time.perf_counter()
     rhs = op.AT(bt, "fp64")
     nrhs = float(torch.linalg.norm(rhs))
     mu, iters = _cg(op, rhs, torch.zeros_like(rhs), tol * nrhs, max_iter, "fp64")
@@ -145,16 +155,19 @@ def cgnr_solve(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: fl
 
 
 def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
-                     max_iter: int = 5000, max_restarts: int = 10, an: float = 1.0):
+                     max_iter: int = 5000, max_restarts: int = 10, an: float = 1.0,
+                     plan: bool = True):
     """CGNR whose operator applications run in FP32c while every vector, dot
     product and update stays FP64; on convergence of the recurrence the
     residual is recomputed with the FP64 operator and exact adjoint, and CG
     restarts from that true residual (residual replacement) until the FP64
-    test passes."""
-    op = NormalOperator(wt, width, an)
+    test passes.  ``plan``: the FP32c adjoint runs through an adjoint plan
+    (its build is inside the timed region)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op = NormalOperator(wt, width, an, plan=plan)
     n = wt.points.shape[0]
     bt = torch.full((n, 3), b, dtype=torch.float64, device=wt.device)
-    t0 = time.perf_counter()
     rhs = op.AT(bt, "fp64")
     nrhs = float(torch.linalg.norm(rhs))
     mu = torch.zeros_like(rhs)

@@ -191,6 +191,34 @@ class WindingNumberTreecode:
                                             _ptr(out), _stream(self.device)))
         return out
 
+    def adjoint_plan(self, weights: torch.Tensor) -> None:
+        """Record the far-field step sequence of the FP32 adjoint for the node
+        centres of ``weights`` [N] (an iterative solve applies the adjoint many
+        times with the same centres); ``apply_adjoint_planned`` then runs
+        without atomics and deterministically."""
+        n = self.points.shape[0]
+        weights = weights.contiguous()
+        assert weights.shape == (n,)
+        with torch.cuda.device(self.device):
+            check(self.L.cagr_adjoint_plan(self._h, _ptr(weights), _dt(weights),
+                                           _stream(self.device)))
+
+    def apply_adjoint_planned(self, op: str, g: torch.Tensor, widths: torch.Tensor, *,
+                              continuous=False, an: float = 1.0,
+                              precision: Optional[str] = None) -> torch.Tensor:
+        """``apply_adjoint`` with the weights of the adjoint plan."""
+        n = self.points.shape[0]
+        g = g.contiguous()
+        assert g.shape == (n, OUT_DIM[op]), (op, tuple(g.shape))
+        widths = widths.contiguous().to(g.dtype)
+        assert widths.shape == (n,)
+        out = torch.empty((n, 3), dtype=g.dtype, device=self.device)
+        with torch.cuda.device(self.device):
+            check(self.L.cagr_apply_adjoint_planned(self._h, OP[op], self._prec(precision), _ptr(g),
+                                                    _dt(g), _ptr(widths), int(continuous),
+                                                    float(an), _ptr(out), _stream(self.device)))
+        return out
+
     def upward(self, attrs: torch.Tensor, weights: Optional[torch.Tensor] = None):
         """FP64 node moments (node_attrs [M,C], node_reppoints [M,3], node_weights [M])
         as scatter_point_attrs_to_nodes returns them (preorder node ids)."""

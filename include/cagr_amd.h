@@ -128,6 +128,23 @@ int cagr_apply_adjoint(cagr_tree* tree, int op, int precision, const void* g, in
                        const void* weights, const void* widths, int continuous, double an,
                        void* out, void* stream);
 
+/* Adjoint plan for repeated adjoints with the same node centres (an
+ * iterative solve): records, for `weights` [n] (original order, `dtype`), the
+ * sequence of far-field steps of every wave of the FP32 adjoint walk, so that
+ * cagr_apply_adjoint_planned stores each step's wave sum into a fixed slot
+ * and sums every node's slots in a fixed order, with no atomics.  Keeps a
+ * copy of the weights; rebuilding replaces the previous plan.  Single-cloud
+ * trees.  No counterpart in the reference (its operators recompute
+ * everything per call). */
+int cagr_adjoint_plan(cagr_tree* tree, const void* weights, int dtype, void* stream);
+
+/* cagr_apply_adjoint with the weights of the tree's adjoint plan
+ * (CAGR_ERR_STATE without a plan).  FP32C runs through the plan and is
+ * deterministic; FP64 is the plain adjoint. */
+int cagr_apply_adjoint_planned(cagr_tree* tree, int op, int precision, const void* g, int dtype,
+                               const void* widths, int continuous, double an, void* out,
+                               void* stream);
+
 /* Operator at external queries [nq,3] (qdtype) -- e.g. the GaussRecon sample
  * isovalue pass (main_GaussRecon_cuda.cu:450-467).  widths [nq] (dtype) or
  * NULL for the scalar `width`.  Queries are Morton-ordered internally for the

@@ -4,9 +4,12 @@
   against central finite differences in FP64 (h = 1e-4 on 64 random entries;
   the operator is linear, so the differences are exact up to rounding) and
   against the explicit transpose of the GPU forward at small N.
-* CGNR: the FP64 solve reaches ||A^T(b - A mu)|| / ||A^T b|| <= 1e-6 in FP64;
-  the mixed-precision solve (FP32c iterations, FP64 refinement and stopping
-  test) reaches the same residual and the same mu.
+* planned adjoint: the adjoint plan (fixed far-step slots, no atomics) gives
+  the same result as the atomic adjoint, bit for bit from call to call.
+* CGNR: the FP64 solve and the mixed-precision solve (FP32c iterations
+  through the adjoint plan, FP64 refinement and stopping test) both reach
+  ||A^T(b - A mu)|| / ||A^T b|| <= 1e-3, checked again with the FP64
+  operator.
 """
 import numpy as np
 import pytest
@@ -79,6 +82,59 @@ def test_autograd_adjoint_is_explicit_transpose():
     assert float((adj - ref).abs().max() / ref.abs().max()) <= 1e-12
     adj32 = wt.apply_adjoint("A2", g, widths, w1, precision="fp32c").reshape(-1)
     assert float((adj32 - ref).abs().max() / ref.abs().max()) <= 2e-6
+    # the planned adjoint is the same transpose
+    wt.adjoint_plan(w1)
+    adjp = wt.apply_adjoint_planned("A2", g, widths, precision="fp32c").reshape(-1)
+    assert float((adjp - ref).abs().max() / ref.abs().max()) <= 2e-6
+
+
+def rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+@pytest.mark.parametrize("op", ["A2", "A3", "A", "A22"])
+def test_planned_adjoint_matches_atomic_and_is_deterministic(op):
+    say("planned adjoint", op)
+    pts = cloud(30000, 7)
+    n = pts.shape[0]
+    wt = WindingNumberTreecode(pts, leaf_size=32)
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    w1 = torch.ones(n, dtype=torch.float64, device=DEV)
+    widths = torch.full((n,), 0.01, dtype=torch.float64, device=DEV)
+    g = torch.randn(n, 3 if op == "A2" else 1, generator=gen, dtype=torch.float64).to(DEV)
+    ref = wt.apply_adjoint(op, g, widths, w1, precision="fp32c")
+    wt.adjoint_plan(w1)
+    a = wt.apply_adjoint_planned(op, g, widths, precision="fp32c")
+    b = wt.apply_adjoint_planned(op, g, widths, precision="fp32c")
+    assert torch.equal(a, b)                  # no atomics on the far field: bitwise repeatable
+    assert rel(a, ref) <= 1e-6                # same FP32 wave sums, FP64 totals in another order
+    # forward applies in between overwrite the node records; the plan keeps its own weights
+    mu = torch.randn(n, 3, generator=gen, dtype=torch.float64).to(DEV)
+    wt.apply("A2", mu, widths, precision="fp32c")
+    c = wt.apply_adjoint_planned(op, g, widths, precision="fp32c")
+    assert torch.equal(a, c)
+    # FP64 with the plan's weights is the plain FP64 adjoint
+    r64 = wt.apply_adjoint(op, g, widths, w1, precision="fp64")
+    p64 = wt.apply_adjoint_planned(op, g, widths, precision="fp64")
+    assert rel(p64, r64) <= 1e-12
+    assert rel(a, r64) <= 2e-6
+
+
+def test_planned_adjoint_weights_and_rebuild():
+    say("planned adjoint weights")
+    pts = cloud(8000, 9)
+    n = pts.shape[0]
+    wt = WindingNumberTreecode(pts, leaf_size=16)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    widths = torch.full((n,), 0.02, dtype=torch.float32, device=DEV)
+    g = torch.randn(n, 3, generator=gen, dtype=torch.float32).to(DEV)
+    for seed in (1, 2):   # non-uniform weights (their centres differ from the unit ones); rebuild
+        wts = (0.5 + torch.rand(n, generator=torch.Generator().manual_seed(seed))).to(DEV)
+        ref = wt.apply_adjoint("A2", g, widths, wts, precision="fp32c")
+        wt.adjoint_plan(wts)
+        got = wt.apply_adjoint_planned("A2", g, widths, precision="fp32c")
+        assert got.dtype == torch.float32
+        assert rel(got.double(), ref.double()) <= 2e-6
 
 
 def test_cgnr_fp64_and_mixed_reach_tolerance():
