@@ -25,6 +25,15 @@ timed inside the solve; also reported without the plan); all start from
 mu = 0 on the tree built above.
 
 Prints one JSON line.
+
+The other configurations of the metric's table run on request, each printing
+one JSON line of its own (they are parity cases of the test suite, measured
+here at full size): --config c1 (10k sphere: the reference's 40-iteration
+schedule on the FP64 oracle on the host cores and on the GPU, 128^3 grid
+query), c2 (100k torus with 5 % outliers: FP64 drop-in matvec pairs, 256^3
+grid query), c4 (500k displaced sphere: autograd forward + backward, FP64
+finite-difference check on 64 entries), c5 (64 clouds, ~3M points: autotuned
+batched persistent launch against one launch per cloud).
 """
 from __future__ import annotations
 
@@ -63,10 +72,30 @@ def parse():
     ap.add_argument("--solve-tol", type=float, default=1e-3,
                     help="FP64-verified relative residual of the solve comparison")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c3 (default): the headline workload; others: see the module docstring")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times and exit (calibrates "
                          "the HBM byte counters FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = "profiles/pmc_traffic_1M_ellipsoids.md"
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), for the default workload only."""
+    if (args.points, args.leaf, args.theta, args.width, args.seed) != (1_000_000, 32, 0.5, 0.002, 0):
+        return None
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    try:
+        text = open(path).read()
+        blob = text[text.index("```json") + 7:]
+        data = json.loads(blob[:blob.index("```")])
+        return data["traffic_bytes"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def log(*a):
@@ -94,6 +123,10 @@ def main():
     from cagr_amd import WindingNumberTreecode, synth
 
     dev = torch.device("cuda:0")
+    if args.config != "c3":
+        line = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5}[args.config](args)
+        print(json.dumps(line), flush=True)
+        return
     n = args.points
     log(f"generating {n} points")
     pts = synth.normalize(synth.ellipsoids(n, args.seed)).astype(np.float32)
@@ -151,6 +184,7 @@ def main():
     flop_a = inter_a * 3 * F_OP["A2k"]          # three directions per interaction
     flop_t = inter_t * F_OP["A2Tk"]             # interactions already summed over the 3 sets
     dom = "A2" if t_a2 >= t_a2t else "A2T_sum"
+    dom_kernel = "k_trav_v<1, 2, 0, false, false>" if dom == "A2" else "k_trav_s<1, 3, 0, false>"
     dom_flop, dom_ms = (flop_a, t_a2) if dom == "A2" else (flop_t, t_a2t)
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
     log(f"interactions per launch: A2 {inter_a:.3e}, A2T_sum {inter_t:.3e}; dominant {dom}: "
@@ -265,12 +299,13 @@ def main():
         "build_ms": round(build_ms, 3),
         "roofline": {
             "bound": "valu",
-            "kernel": f"k_trav_{'v' if dom == 'A2' else 's'} ({dom}, fp32c)",
+            "kernel": f"{dom_kernel} ({dom}, fp32c)",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_VECTOR_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_VECTOR_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom_kernel, args),
+            "traffic_source": TRAFFIC_FILE,
             "flop_per_launch": dom_flop,
             "launch_ms": round(float(dom_ms), 4),
         },
@@ -345,6 +380,219 @@ def cpu_baseline(pts, mu_np, s3, args, r32, a32):
                   f"oracle tree build {t_build:.1f} s not included",
         "gpu_fp32c_err_vs_oracle_on_sample": {"A_mu": ea, "AT_r": er},
     }
+
+
+def _sync_time(fn, reps=1):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps, out
+
+
+def _pairs_per_s(wt, mu, w, precision, steps):
+    def step():
+        a = wt.forward_A2(mu, w, precision=precision)
+        return wt.forward_A2T_sum(-a, w, precision=precision)
+    step()
+    dt, _ = _sync_time(step, steps)
+    return 1.0 / dt
+
+
+def config_c1(args):
+    """10k sphere (sigma 0.002), leaf 32, theta 0.5: the reference's schedule
+    (main_CAGR.py:90-174, preset l0, An = 1) on the host cores with the FP64
+    oracle and on the GPU in both precisions; 128^3 grid query."""
+    import torch
+    import oracle
+    from oracle import solve_ref
+    from cagr_amd import WindingNumberTreecode, synth
+    from cagr_amd.query import grid_values, sample_attributes
+    from cagr_amd.solve import PRESETS, cagr_schedule
+    dev = torch.device("cuda:0")
+    n = 10_000
+    pts = synth.normalize(synth.sphere(n, args.seed, 0.002)).astype(np.float32)
+    wsmin, wsmax = PRESETS["l0"]
+    out = {"metric": "schedule seconds (40 iterations + 8 G steps) and 128^3 grid query seconds",
+           "config": {"workload": "10k points on the unit sphere, sigma 0.002, leaf 32, theta 0.5, "
+                                  "preset l0, An 1", "points": n}, "data": "synthetic"}
+    pt = torch.from_numpy(pts).to(dev)
+    res = {}
+    for prec, dt in (("fp64", torch.float64), ("fp32c", torch.float32)):
+        wt = WindingNumberTreecode(pt.to(dt), leaf_size=32, theta=0.5, precision=prec)
+        cagr_schedule(wt, wsmin, wsmax, precision=prec)   # warm-up
+        sec, (normals, rn) = _sync_time(lambda: cagr_schedule(wt, wsmin, wsmax, precision=prec))
+        mu = sample_attributes(normals)
+        grid_values(wt, mu, [-1.2] * 3, [1.2] * 3, [128] * 3, 0.01, precision=prec)
+        gsec, vals = _sync_time(lambda: grid_values(wt, mu, [-1.2] * 3, [1.2] * 3, [128] * 3, 0.01,
+                                                    precision=prec))
+        res[prec] = {"schedule_s": round(sec, 4), "grid128_s": round(gsec, 4),
+                     "normals": normals.double().cpu().numpy()}
+        log(f"c1 {prec}: schedule {sec:.3f} s, 128^3 grid {gsec * 1e3:.1f} ms")
+    L = oracle.lib()
+    cores = int(L.orc_num_threads())
+    t = time.perf_counter()
+    ref_normals, _ = solve_ref.cagr_schedule(pts.astype(np.float64), wsmin, wsmax, leaf_size=32,
+                                             theta=0.5)
+    cpu_s = time.perf_counter() - t
+    log(f"c1 cpu oracle schedule {cpu_s:.2f} s on {cores} threads")
+    def diff(a):
+        return float(np.abs(a - ref_normals).max() / np.abs(ref_normals).max())
+    out["gpu"] = {k: {kk: vv for kk, vv in v.items() if kk != "normals"} for k, v in res.items()}
+    out["normals_err_vs_oracle"] = {k: diff(v["normals"]) for k, v in res.items()}
+    out["cpu_baseline"] = {"value": round(cpu_s, 3), "unit": "s (schedule)", "cores": cores,
+                           "kind": "port", "sample": "the whole 40-iteration schedule + 8 G steps"}
+    out["speedup_fp32c_vs_cpu"] = round(cpu_s / res["fp32c"]["schedule_s"], 1)
+    out["speedup_fp32c_vs_fp64"] = round(res["fp64"]["schedule_s"] / res["fp32c"]["schedule_s"], 3)
+    return out
+
+
+def config_c2(args):
+    """100k torus (R 1, r 0.3, sigma 0.001, 5 % uniform outliers), leaf 32,
+    theta 0.5: FP64 drop-in matvec pairs/s (and FP32c), 256^3 grid query."""
+    import torch
+    from cagr_amd import WindingNumberTreecode, synth
+    from cagr_amd.query import grid_values, sample_attributes
+    dev = torch.device("cuda:0")
+    n = 100_000
+    pts = synth.normalize(synth.torus(n, args.seed)).astype(np.float32)
+    mu_np = synth.random_normals(n, args.seed + 1)
+    out = {"metric": "matvec pairs/s and 256^3 grid query seconds",
+           "config": {"workload": "100k torus points, sigma 0.001, 5% outliers, leaf 32, theta 0.5, "
+                                  "w 0.002 (pairs), w 0.01 (grid)", "points": n},
+           "data": "synthetic"}
+    for prec, dt in (("fp64", torch.float64), ("fp32c", torch.float32)):
+        pt = torch.from_numpy(pts).to(dev).to(dt)
+        wt = WindingNumberTreecode(pt, leaf_size=32, theta=0.5, precision=prec)
+        mu = torch.from_numpy(mu_np).to(dev).to(dt)
+        w = torch.full((n,), 0.002, dtype=dt, device=dev)
+        pps = _pairs_per_s(wt, mu, w, prec, 10)
+        att = sample_attributes(mu)
+        grid_values(wt, att, [-1.2] * 3, [1.2] * 3, [64] * 3, 0.01, precision=prec)
+        gsec, vals = _sync_time(lambda: grid_values(wt, att, [-1.2] * 3, [1.2] * 3, [256] * 3, 0.01,
+                                                    precision=prec))
+        out[prec] = {"pairs_per_s": round(pps, 3), "grid256_s": round(gsec, 4),
+                     "grid256_queries_per_s": round(256 ** 3 / gsec, 1)}
+        if prec == "fp64":
+            g64 = vals.double()
+        else:
+            out[prec]["grid_err_vs_fp64"] = float((vals.double() - g64).abs().max() / g64.abs().max())
+        log(f"c2 {prec}: {pps:.1f} pairs/s, 256^3 grid {gsec:.3f} s")
+    out["speedup_fp32c_vs_fp64"] = {"pairs": round(out["fp32c"]["pairs_per_s"] / out["fp64"]["pairs_per_s"], 3),
+                                    "grid": round(out["fp64"]["grid256_s"] / out["fp32c"]["grid256_s"], 3)}
+    return out
+
+
+def config_c4(args):
+    """500k displaced sphere: loss = ||A2 mu - target||^2 through the autograd
+    form; forward + backward time in FP32c and FP64; FP64 gradient against
+    central differences (h = 1e-4) on 64 random entries; FP32c gradient
+    against the FP64 one."""
+    import torch
+    from cagr_amd import WindingNumberTreecode, synth
+    from cagr_amd.autograd import treecode_op
+    dev = torch.device("cuda:0")
+    n = 500_000
+    pts = synth.normalize(synth.displaced_sphere(n, args.seed)).astype(np.float32)
+    gen = torch.Generator(device="cpu").manual_seed(args.seed + 3)
+    mu0 = torch.randn(n, 3, generator=gen, dtype=torch.float64).to(dev)
+    target = torch.randn(n, 3, generator=gen, dtype=torch.float64).to(dev)
+    out = {"metric": "autograd forward+backward seconds; gradient error",
+           "config": {"workload": "500k displaced-sphere points, leaf 32, theta 0.5, w 0.002, A2",
+                      "points": n}, "data": "synthetic"}
+    grads = {}
+    for prec, dt in (("fp64", torch.float64), ("fp32c", torch.float32)):
+        wt = WindingNumberTreecode(torch.from_numpy(pts).to(dev).to(dt), leaf_size=32, theta=0.5,
+                                   precision=prec)
+        widths = torch.full((n,), 0.002, dtype=dt, device=dev)
+
+        def fb():
+            mu = mu0.to(dt).clone().requires_grad_(True)
+            y = treecode_op(wt, mu, widths, op="A2", precision=prec)
+            loss = ((y - target.to(dt)) ** 2).sum()
+            loss.backward()
+            return mu.grad
+        fb()
+        sec, g = _sync_time(fb, 3)
+        grads[prec] = g.double()
+        out[prec] = {"fwd_bwd_s": round(sec, 5)}
+        log(f"c4 {prec}: forward+backward {sec * 1e3:.2f} ms")
+        if prec == "fp64":
+            def loss_fn(m):
+                y = treecode_op(wt, m, widths, op="A2", precision="fp64")
+                return float(((y - target) ** 2).sum())
+            h = 1e-4
+            idx = np.random.default_rng(args.seed + 4).choice(n * 3, 64, replace=False)
+            fd = []
+            with torch.no_grad():
+                for k in idx:
+                    e = torch.zeros(n * 3, dtype=torch.float64, device=dev)
+                    e[k] = h
+                    e = e.view(n, 3)
+                    fd.append((loss_fn(mu0 + e) - loss_fn(mu0 - e)) / (2 * h))
+            fd = np.array(fd)
+            got = g.double().cpu().numpy().reshape(-1)[idx]
+            out["fp64"]["fd_rel_err"] = float(np.abs(got - fd).max() / np.abs(fd).max())
+    out["fp32c"]["grad_err_vs_fp64"] = float((grads["fp32c"] - grads["fp64"]).abs().max()
+                                             / grads["fp64"].abs().max())
+    out["speedup_fp32c_vs_fp64"] = round(out["fp64"]["fwd_bwd_s"] / out["fp32c"]["fwd_bwd_s"], 3)
+    return out
+
+
+def config_c5(args):
+    """64 clouds (sizes log-uniform in [2e3, 2e5], sphere / torus / ellipsoid,
+    sigma 0.001): autotune leaf {16, 32, 64} x theta {0.4, 0.5, 0.6} x waves
+    per block {1, 2, 4, 8}, then the tuned batched persistent form against one
+    single-cloud launch pair per cloud at leaf 32, theta 0.5."""
+    import torch
+    from cagr_amd import WindingNumberTreecode, synth
+    from cagr_amd.autotune import autotune
+    from cagr_amd.batched import BatchedTreecode
+    dev = torch.device("cuda:0")
+    clouds_np = synth.batch_clouds(64, args.seed)
+    clouds = [torch.from_numpy(c.astype(np.float32)).to(dev) for c in clouds_np]
+    mus = [torch.from_numpy(synth.random_normals(c.shape[0], args.seed + i).astype(np.float32)).to(dev)
+           for i, c in enumerate(clouds_np)]
+    total = int(sum(c.shape[0] for c in clouds_np))
+    rep = autotune(clouds, mus, 0.002, steps=3, log=log)
+    b = rep.best
+    log(f"c5 tuned: leaf {b.leaf_size}, theta {b.theta}, waves/block {b.waves_per_block}: "
+        f"{b.ms_per_pair:.3f} ms/pair (error {b.err:.2e}, budget {rep.budget:.2e})")
+    mu = torch.cat(mus).contiguous()
+    bt = BatchedTreecode(clouds, leaf_size=b.leaf_size, theta=b.theta, precision="fp32c")
+    bt.balance("A2", mu)
+    bt.set_launch(b.waves_per_block)
+    w = torch.full((total,), 0.002, dtype=torch.float32, device=dev)
+    tuned = _pairs_per_s(bt, mu, w, "fp32c", 10)
+    bd = BatchedTreecode(clouds, leaf_size=32, theta=0.5, precision="fp32c")
+    bd.balance("A2", mu)
+    default = _pairs_per_s(bd, mu, w, "fp32c", 10)
+    singles = [WindingNumberTreecode(c, leaf_size=32, theta=0.5, precision="fp32c") for c in clouds]
+    ws = [torch.full((c.shape[0],), 0.002, dtype=torch.float32, device=dev) for c in clouds]
+
+    def all_singles():
+        for wt, m, wi in zip(singles, mus, ws):
+            a = wt.forward_A2(m, wi, precision="fp32c")
+            wt.forward_A2T_sum(-a, wi, precision="fp32c")
+    all_singles()
+    sec, _ = _sync_time(all_singles, 5)
+    per_cloud = 1.0 / sec
+    log(f"c5: tuned batched {tuned:.1f} pairs/s, batched leaf 32 {default:.1f}, "
+        f"one launch pair per cloud {per_cloud:.1f}")
+    return {"metric": "batched matvec pairs/s (all 64 clouds per pair)",
+            "config": {"workload": f"64 clouds, {total} points, w 0.002, A2 + A2T_sum, FP32c",
+                       "points": total},
+            "data": "synthetic",
+            "tuned": {"leaf_size": b.leaf_size, "theta": b.theta,
+                      "waves_per_block": b.waves_per_block, "pairs_per_s": round(tuned, 3),
+                      "sampled_err": b.err, "err_budget": rep.budget},
+            "batched_leaf32_theta05": round(default, 3),
+            "per_cloud_launches": round(per_cloud, 3),
+            "speedup_tuned_vs_per_cloud": round(tuned / per_cloud, 3),
+            "table": [[r.leaf_size, r.theta, r.waves_per_block, round(r.ms_per_pair, 4), r.err]
+                      for r in rep.table]}
 
 
 if __name__ == "__main__":
