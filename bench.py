@@ -264,6 +264,9 @@ def main():
             "mixed": {"iters": i32["iters"], "restarts": i32["restarts"],
                       "seconds": round(i32["seconds"], 4), "rel_residual": i32["rel_residual"]},
             "speedup": round(i64["seconds"] / i32["seconds"], 3),
+            "build_s": round(build_ms / 1e3, 5),
+            "speedup_with_build": round((i64["seconds"] + build_ms / 1e3)
+                                        / (i32["seconds"] + build_ms / 1e3), 3),
             "mixed_atomic_adjoint": {"iters": ia["iters"], "seconds": round(ia["seconds"], 4),
                                      "speedup": round(i64["seconds"] / ia["seconds"], 3)},
             "operator_ms": op_ms,
@@ -356,12 +359,16 @@ def cpu_baseline(pts, mu_np, s3, args, r32, a32):
                                node_moments=mom_s[k], theta=args.theta) for k in range(3)]
         return time.perf_counter() - t, np.concatenate(outs, 1), sum(outs_t)
 
-    q0 = rng.choice(n, 256, replace=False)
-    dt0, _, _ = run(q0)
-    per_q = dt0 / 256
-    m = int(min(n, max(256, args.cpu_seconds / max(per_q, 1e-9))))
-    q_idx = np.sort(rng.choice(n, m, replace=False))
-    dt, ref_a, ref_r = run(q_idx)
+    # size the sample to about --cpu-seconds of host work: a pilot, then grow
+    # the sample until one run takes at least half the target (fixed per-call
+    # costs make small pilots overestimate the per-query time)
+    m = 4096
+    while True:
+        q_idx = np.sort(rng.choice(n, m, replace=False))
+        dt, ref_a, ref_r = run(q_idx)
+        if dt >= 0.5 * args.cpu_seconds or m >= n:
+            break
+        m = int(min(n, max(2 * m, m * args.cpu_seconds / max(dt, 1e-3))))
     per_pair = t_up + dt * n / m
     # accuracy of the GPU FP32c result on the sampled queries against the FP64 oracle
     ga = a32.double().cpu().numpy()[q_idx]
