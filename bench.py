@@ -11,7 +11,8 @@ One step = one matvec pair: A.mu (A21, A22, A23 on mu [N,3], one fused
 upward pass + traversal) followed by A^T.r (A2T1, A2T2, A2T3 on the three
 columns of r = -A.mu, summed; one fused pass with three sets of centres), with
 the inputs resident in HBM.  Upward passes are inside the step; the tree
-build is outside (reported as build_ms).  The main figure is the
+build and the ordering of the traversal waves (longest first, from exact
+interaction counts) are outside (build_ms, balance_ms).  The main figure is the
 FP32-compensated form; the plain FP64 operator on the same workload is timed
 beside it (fp64_value), and the oracle (the FP64 CPU restatement) is timed on
 a bounded sample of queries on the host cores (cpu_baseline).
@@ -142,6 +143,13 @@ def main():
     build_ms = (time.perf_counter() - t0) * 1e3
     info = wt.info()
     log(f"tree built in {build_ms:.1f} ms: {info}")
+    # dispatch order of the traversal waves, longest first, costed by the exact
+    # interaction counts of A2 on mu (setup, like the build; outside the timed steps)
+    t0 = time.perf_counter()
+    wt.balance("A2", mu)
+    torch.cuda.synchronize()
+    balance_ms = (time.perf_counter() - t0) * 1e3
+    log(f"waves ordered longest first in {balance_ms:.1f} ms")
 
     def step(prec):
         a = wt.forward_A2(mu, w, precision=prec)
@@ -264,9 +272,9 @@ def main():
             "mixed": {"iters": i32["iters"], "restarts": i32["restarts"],
                       "seconds": round(i32["seconds"], 4), "rel_residual": i32["rel_residual"]},
             "speedup": round(i64["seconds"] / i32["seconds"], 3),
-            "build_s": round(build_ms / 1e3, 5),
-            "speedup_with_build": round((i64["seconds"] + build_ms / 1e3)
-                                        / (i32["seconds"] + build_ms / 1e3), 3),
+            "build_s": round((build_ms + balance_ms) / 1e3, 5),
+            "speedup_with_build": round((i64["seconds"] + (build_ms + balance_ms) / 1e3)
+                                        / (i32["seconds"] + (build_ms + balance_ms) / 1e3), 3),
             "mixed_atomic_adjoint": {"iters": ia["iters"], "seconds": round(ia["seconds"], 4),
                                      "speedup": round(i64["seconds"] / ia["seconds"], 3)},
             "operator_ms": op_ms,
@@ -300,6 +308,7 @@ def main():
             "nodes": info["num_nodes"], "depth": info["depth"],
         },
         "build_ms": round(build_ms, 3),
+        "balance_ms": round(balance_ms, 3),
         "roofline": {
             "bound": "valu",
             "kernel": f"{dom_kernel} ({dom}, fp32c)",

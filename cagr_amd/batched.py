@@ -66,13 +66,3 @@ class BatchedTreecode(WindingNumberTreecode):
     def set_launch(self, waves_per_block: int = 4, grid_blocks: int = 0) -> None:
         check(self.L.cagr_set_launch(self._h, 1, int(waves_per_block), int(grid_blocks)))
 
-    def balance(self, op: str, attrs: torch.Tensor) -> np.ndarray:
-        """Order the work items longest first by the exact interaction counts of
-        `op` on `attrs`; returns the per-query cost used."""
-        far, near, _ = self.counts(op, attrs)
-        cost = (far + near).cpu().numpy().astype(np.int64)
-        check(self.L.cagr_order_items(self._h, cost.ctypes.data_as(ct.c_void_p)))
-        return cost
-
-    def unbalance(self) -> None:
-        check(self.L.cagr_order_items(self._h, None))

@@ -29,15 +29,15 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 
 enum { GM_A = 0, GM_A2K = 1, GM_A3 = 2, GM_A2 = 3 };
-// what a far step of the FP32 kernel does with the wave's summed contribution
-// to the node: FP64 atomic into L (no plan), or a plain store into the wave's
-// next slot (with a plan); the plan is built by a counting and a filling walk
-enum { FM_ATOMIC = 0, FM_SLOT = 1, FM_COUNT = 2, FM_FILL = 3 };
+// modes of the FP32 walk: the plain adjoint (far sums as FP64 atomics into L),
+// and the two walks that build the adjoint plan (count, then record)
+enum { FM_ATOMIC = 0, FM_COUNT = 2, FM_FILL = 3 };
 
 struct AP {
     int64_t n;
     int32_t root, stop;
     const int32_t* perm;
+    const int32_t* tile_order;// dispatch slot -> wave (null: identity); one wave per block
     const void* g;            // cotangent [n, gcols], original order
     int gdtype;
     int gmode;                // GM_*
@@ -57,7 +57,21 @@ struct AP {
     uint64_t* slot_node;      // [slots] node of each slot (filling walk)
     int32_t* slot_id;         // [slots] the slot's own index (filling walk)
     float4* far_val;          // [slots] wave sums of the far steps
+    // the walk itself, recorded (filling walk) and replayed (list kernels)
+    int32_t* far_node;        // [slots] node of each far step, wave-major
+    uint64_t* far_mask;       // [slots] lanes that take it
+    const int32_t* near_off;  // [waves + 1] first near-leaf entry of each wave
+    int32_t* near_cnt;        // [waves] near-leaf steps per wave (counting walk)
+    int32_t* near_leaf;       // [entries] leaf of each near step
+    uint64_t* near_mask;      // [entries] lanes near to it
 };
+
+// the wave (64 consecutive sorted points) this block serves: blocks of one
+// wave, dispatched XCD-aware, in the tree's longest-first order when it has one
+__device__ inline int64_t wave_of(const AP& p) {
+    const int64_t slot = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    return p.tile_order ? (int64_t)p.tile_order[slot] : slot;
+}
 
 // cotangent of query oi as the per-direction weights of the k-sum form
 // (GM_A: scalar in gk[0])
@@ -110,6 +124,75 @@ __device__ inline void adj_term32(float* z, const float* d, float r2, bool use, 
     }
 }
 
+// The transposed near block of one leaf for the wave's near lanes qmask:
+// lanes hold the leaf's sources; for leaves of <= 32 points both wave halves
+// hold the same 32 sources and take alternate queries of the mask, so every
+// lane works, and the halves' sums meet through one lane shift.  One FP64
+// atomic per source and component into N.  All 64 lanes must be active.
+template <bool GA, bool CONT>
+__device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qmask, int lane,
+                                     const SrcV32* __restrict__ src, const double* x,
+                                     const float* xf, const float* h, const float* gin, float w2,
+                                     double w, float winv, float ew, float hanf) {
+    const bool split = t.end - t.begin <= 32;
+    const int half = lane >> 5;
+    for (int32_t base = t.begin; base < t.end; base += split ? 32 : 64) {
+        const int32_t j = base + (split ? (lane & 31) : lane);
+        const bool has = j < t.end;
+        const SrcV32 sr = src[has ? j : t.begin];   // lanes past the leaf read a real record
+        float zz[3] = {0.f, 0.f, 0.f};
+        uint64_t mm = qmask;
+        while (mm != 0) {
+            const int qi = __builtin_ctzll(mm);
+            mm &= mm - 1;
+            int qj = qi;       // the upper half's query (split mode)
+            bool qj_ok = !split;
+            if (split && mm != 0) {
+                qj = __builtin_ctzll(mm);
+                mm &= mm - 1;
+                qj_ok = true;
+            }
+            // the lane's query value, fetched through the LDS
+            // crossbar (ds_bpermute; every lane is active here)
+            const int qsrc = (split && half) ? qj : qi;
+            auto pick = [&](auto v) { return __shfl(v, qsrc, 64); };
+            const float qx[3] = {pick(xf[0]), pick(xf[1]), pick(xf[2])};
+            const float qh[3] = {pick(h[0]), pick(h[1]), pick(h[2])};
+            const float qw2 = pick(w2);
+            float qg[3] = {0.f, 0.f, 0.f}, qwinv = 0.f, qew = 0.f;
+            if constexpr (CONT) {
+                qg[0] = pick(gin[0]);
+                qg[1] = pick(gin[1]);
+                qg[2] = pick(gin[2]);
+                qwinv = pick(winv);
+                qew = pick(ew);
+            }
+            const bool use = has && (!split || !half || qj_ok);
+            const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
+            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+            const bool amb = use && fabsf(r2 - qw2) <= 1e-5f * qw2;
+            bool in = r2 < qw2;
+            if (__builtin_expect(__ballot(amb) != 0, 0)) {
+                const double qx64[3] = {pick(x[0]), pick(x[1]), pick(x[2])};
+                const double qw64 = pick(w);
+                if (amb)
+                    in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
+                                         qx64[1] - (double)sr.y[1],
+                                         qx64[2] - (double)sr.y[2])) >= qw64);
+            }
+            adj_term32<GA, CONT>(zz, d, r2, use, in, qh, sr.e, hanf, qg, qwinv, qew);
+        }
+        if (split) {   // upper half adds its sums into the lower half
+            for (int k = 0; k < 3; ++k) zz[k] += __shfl_down(zz[k], 32, 64);
+        }
+        if (has && (!split || !half)) {
+            atomicAdd(p.adj_pt + j, (double)zz[0]);
+            atomicAdd(p.adj_pt + p.n + j, (double)zz[1]);
+            atomicAdd(p.adj_pt + 2 * p.n + j, (double)zz[2]);
+        }
+    }
+}
+
 template <int GMODE_A, bool CONT, int FM>
 __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restrict__ topo,
                                                     const NodeV32* __restrict__ nodes,
@@ -118,14 +201,14 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                                                     const float* __restrict__ pts32) {
     constexpr bool GA = GMODE_A != 0;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t wave = wave_of(p);
     const int64_t s = wave * 64 + lane;
     const bool valid = s < p.n;
     if (__ballot(valid) == 0) return;
     const int64_t sc = valid ? s : 0;
     const double x[3] = {pts64[3 * sc], pts64[3 * sc + 1], pts64[3 * sc + 2]};
     const float xf[3] = {pts32[3 * sc], pts32[3 * sc + 1], pts32[3 * sc + 2]};
-    constexpr bool MATH = FM == FM_ATOMIC || FM == FM_SLOT;   // the walks of the plan do no math
+    constexpr bool MATH = FM == FM_ATOMIC;   // the walks of the plan do no math
     const int64_t oi = p.perm[sc];
     const double w = MATH ? load_as_double(p.widths, p.wdtype, oi) : 1.0;
     double gk[3] = {0.0, 0.0, 0.0};
@@ -152,11 +235,17 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
     // the wave's slots (plan): [slot0, slot0 + kmax)
     int64_t slot0 = 0;
     int32_t kmax = 0;
-    if constexpr (FM == FM_SLOT || FM == FM_FILL) {
+    if constexpr (FM == FM_FILL) {
         slot0 = p.wave_off[wave];
         kmax = p.wave_off[wave + 1] - (int32_t)slot0;
     }
     int32_t k = 0;   // far steps so far
+    int64_t near0 = 0;
+    int32_t knmax = 0, kn = 0;   // near-leaf steps (plan walks)
+    if constexpr (FM == FM_FILL) {
+        near0 = p.near_off[wave];
+        knmax = p.near_off[wave + 1] - (int32_t)near0;
+    }
 
     // every branch is wave-uniform; lanes that do not take a step add zeros
     for (int32_t n = p.root; n < p.stop;) {
@@ -173,9 +262,12 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             if constexpr (FM == FM_COUNT) {
                 if (lane == 0) atomicAdd(p.node_cnt + n, 1);
             } else if constexpr (FM == FM_FILL) {
+                const uint64_t fmask = __ballot(takefar);
                 if (lane == 0 && k < kmax) {
                     p.slot_node[slot0 + k] = (uint64_t)n;
                     p.slot_id[slot0 + k] = (int32_t)(slot0 + k);
+                    p.far_node[slot0 + k] = n;
+                    p.far_mask[slot0 + k] = fmask;
                 }
             } else {
                 const float d[3] = {(float)e0, (float)e1, (float)e2};
@@ -186,10 +278,7 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                 adj_term32<GA, CONT>(z, d, r2, takefar, in, h, nr.e, hanf, gin, winv, ew);
                 const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]),
                             s2 = wave_sum_dpp(z[2]);
-                if constexpr (FM == FM_SLOT) {
-                    if (lane == 0 && k < kmax)
-                        p.far_val[p.slot_pos[slot0 + k]] = make_float4(s0, s1, s2, 0.f);
-                } else if (lane == 0) {
+                if (lane == 0) {
                     atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
                     atomicAdd(p.adj_node + 3 * (int64_t)n + 1, (double)s1);
                     atomicAdd(p.adj_node + 3 * (int64_t)n + 2, (double)s2);
@@ -201,66 +290,17 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         const uint64_t qmask = __ballot(nearl);
         if (topo_leaf(t)) {
             resume = nearl ? t.escape : resume;
-            if (MATH && qmask != 0) {
-                // Leaves of <= 32 points: both wave halves hold the same 32
-                // sources and take alternate queries of the mask, so every
-                // lane works; the halves' sums meet through one lane shift.
-                const bool split = t.end - t.begin <= 32;
-                const int half = lane >> 5;
-                for (int32_t base = t.begin; base < t.end; base += split ? 32 : 64) {
-                    const int32_t j = base + (split ? (lane & 31) : lane);
-                    const bool has = j < t.end;
-                    const SrcV32 sr = src[has ? j : t.begin];   // lanes past the leaf read a real record
-                    float zz[3] = {0.f, 0.f, 0.f};
-                    uint64_t mm = qmask;
-                    while (mm != 0) {
-                        const int qi = __builtin_ctzll(mm);
-                        mm &= mm - 1;
-                        int qj = qi;       // the upper half's query (split mode)
-                        bool qj_ok = !split;
-                        if (split && mm != 0) {
-                            qj = __builtin_ctzll(mm);
-                            mm &= mm - 1;
-                            qj_ok = true;
+            if (MATH && qmask != 0)
+                adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
+            if constexpr (FM == FM_COUNT || FM == FM_FILL) {
+                if (qmask != 0) {
+                    if constexpr (FM == FM_FILL) {
+                        if (lane == 0 && kn < knmax) {
+                            p.near_leaf[near0 + kn] = n;
+                            p.near_mask[near0 + kn] = qmask;
                         }
-                        // the lane's query value, fetched through the LDS
-                        // crossbar (ds_bpermute; every lane is active here)
-                        const int qsrc = (split && half) ? qj : qi;
-                        auto pick = [&](auto v) { return __shfl(v, qsrc, 64); };
-                        const float qx[3] = {pick(xf[0]), pick(xf[1]), pick(xf[2])};
-                        const float qh[3] = {pick(h[0]), pick(h[1]), pick(h[2])};
-                        const float qw2 = pick(w2);
-                        float qg[3] = {0.f, 0.f, 0.f}, qwinv = 0.f, qew = 0.f;
-                        if constexpr (CONT) {
-                            qg[0] = pick(gin[0]);
-                            qg[1] = pick(gin[1]);
-                            qg[2] = pick(gin[2]);
-                            qwinv = pick(winv);
-                            qew = pick(ew);
-                        }
-                        const bool use = has && (!split || !half || qj_ok);
-                        const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
-                        const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                        const bool amb = use && fabsf(r2 - qw2) <= 1e-5f * qw2;
-                        bool in = r2 < qw2;
-                        if (__builtin_expect(__ballot(amb) != 0, 0)) {
-                            const double qx64[3] = {pick(x[0]), pick(x[1]), pick(x[2])};
-                            const double qw64 = pick(w);
-                            if (amb)
-                                in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
-                                                     qx64[1] - (double)sr.y[1],
-                                                     qx64[2] - (double)sr.y[2])) >= qw64);
-                        }
-                        adj_term32<GA, CONT>(zz, d, r2, use, in, qh, sr.e, hanf, qg, qwinv, qew);
                     }
-                    if (split) {   // upper half adds its sums into the lower half
-                        for (int k = 0; k < 3; ++k) zz[k] += __shfl_down(zz[k], 32, 64);
-                    }
-                    if (has && (!split || !half)) {
-                        atomicAdd(p.adj_pt + j, (double)zz[0]);
-                        atomicAdd(p.adj_pt + p.n + j, (double)zz[1]);
-                        atomicAdd(p.adj_pt + 2 * p.n + j, (double)zz[2]);
-                    }
+                    ++kn;
                 }
             }
             n = t.escape;
@@ -269,7 +309,98 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         }
     }
     if constexpr (FM == FM_COUNT)
-        if (lane == 0) p.wave_cnt[wave] = k;
+        if (lane == 0) {
+            p.wave_cnt[wave] = k;
+            p.near_cnt[wave] = kn;
+        }
+}
+
+// The planned adjoint: the recorded walk replayed.  Far steps come from the
+// wave's list (node, lanes) and need no opening test; their wave sums go to
+// the slots; near leaves come from the wave's near list.
+template <int GMODE_A, bool CONT>
+__global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __restrict__ topo,
+                                                         const NodeV32* __restrict__ nodes,
+                                                         const SrcV32* __restrict__ src,
+                                                         const double* __restrict__ pts64,
+                                                         const float* __restrict__ pts32) {
+    constexpr bool GA = GMODE_A != 0;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = __builtin_amdgcn_readfirstlane((int)wave_of(p));
+    const int64_t s = wave * 64 + lane;
+    const bool valid = s < p.n;
+    if (__ballot(valid) == 0) return;
+    const int64_t sc = valid ? s : 0;
+    const double x[3] = {pts64[3 * sc], pts64[3 * sc + 1], pts64[3 * sc + 2]};
+    const float xf[3] = {pts32[3 * sc], pts32[3 * sc + 1], pts32[3 * sc + 2]};
+    const int64_t oi = p.perm[sc];
+    const double w = load_as_double(p.widths, p.wdtype, oi);
+    double gk[3];
+    load_cot(p, oi, gk);
+    if (!valid) gk[0] = gk[1] = gk[2] = 0.0;
+    float h[3], gin[3];
+    if constexpr (GA) {
+        h[0] = (float)(p.sign * gk[0]);
+        h[1] = h[2] = 0.f;
+        gin[0] = h[0];
+        gin[1] = gin[2] = 0.f;
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            h[c] = (float)(gk[c] * exp(0.5 * p.an * x[c]));
+            gin[c] = (float)gk[c];
+        }
+    }
+    const float wf = (float)w, w2 = wf * wf, winv = 1.f / wf;
+    const float ew = CONT ? (float)exp(p.an * w) : 0.f;
+    const float hanf = (float)(0.5 * p.an);
+    const int dsafe = __builtin_amdgcn_readfirstlane(
+        far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
+
+    // the lists are read 64 entries at a time, one per lane (coalesced), and
+    // entry k is taken from lane k: no dependent load in front of each step
+    const int32_t slot0 = __builtin_amdgcn_readfirstlane(p.wave_off[wave]);
+    const int32_t nf = __builtin_amdgcn_readfirstlane(p.wave_off[wave + 1]) - slot0;
+    int32_t node_l = 0, pos_l = 0;
+    uint64_t mask_l = 0;
+    for (int32_t k = 0; k < nf; ++k) {
+        if ((k & 63) == 0) {
+            const int32_t kl = k + lane;
+            node_l = kl < nf ? p.far_node[slot0 + kl] : 0;
+            mask_l = kl < nf ? p.far_mask[slot0 + kl] : 0;
+            pos_l = kl < nf ? p.slot_pos[slot0 + kl] : 0;
+        }
+        const int32_t n = __builtin_amdgcn_readlane(node_l, k & 63);
+        const uint64_t m = lane_bcast_u64(mask_l, k & 63);
+        const int32_t pos = __builtin_amdgcn_readlane(pos_l, k & 63);
+        const bool take = (m >> lane) & 1;
+        const TopoRec t = rec32(topo, n);
+        const NodeV32 nr = rec32(nodes, n);
+        const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
+        const double d2 = dist2_rn(e0, e1, e2);
+        const float d[3] = {(float)e0, (float)e1, (float)e2};
+        const float r2 = (float)d2;
+        bool in = false;   // far at depth <= dsafe proves r > w
+        if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, take);
+        float z[3] = {0.f, 0.f, 0.f};
+        adj_term32<GA, CONT>(z, d, r2, take, in, h, nr.e, hanf, gin, winv, ew);
+        const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]), s2 = wave_sum_dpp(z[2]);
+        if (lane == 0) p.far_val[pos] = make_float4(s0, s1, s2, 0.f);
+    }
+    const int32_t near0 = __builtin_amdgcn_readfirstlane(p.near_off[wave]);
+    const int32_t nn = __builtin_amdgcn_readfirstlane(p.near_off[wave + 1]) - near0;
+    int32_t leaf_l = 0;
+    uint64_t nmask_l = 0;
+    for (int32_t j = 0; j < nn; ++j) {
+        if ((j & 63) == 0) {
+            const int32_t jl = j + lane;
+            leaf_l = jl < nn ? p.near_leaf[near0 + jl] : 0;
+            nmask_l = jl < nn ? p.near_mask[near0 + jl] : 0;
+        }
+        const int32_t n = __builtin_amdgcn_readlane(leaf_l, j & 63);
+        const uint64_t qmask = lane_bcast_u64(nmask_l, j & 63);
+        const TopoRec t = rec32(topo, n);
+        adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
+    }
 }
 
 // L of every node from its run of slots, one wave per node: lanes stride the
@@ -358,7 +489,7 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
                                                     const double* __restrict__ pts64) {
     constexpr bool GA = GMODE_A != 0;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t wave = wave_of(p);
     const int64_t s = wave * 64 + lane;
     const bool valid = s < p.n;
     if (__ballot(valid) == 0) return;
@@ -482,23 +613,29 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
         p.wave_off = t->plan_wave_off;
         p.slot_pos = t->plan_slot_pos;
         p.far_val = t->plan_far_val;
+        p.far_node = t->plan_far_node;
+        p.far_mask = t->plan_far_mask;
+        p.near_off = t->plan_near_off;
+        p.near_leaf = t->plan_near_leaf;
+        p.near_mask = t->plan_near_mask;
     } else {
         k_fill0<<<(3 * m + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_node, 3 * m);
         CAGR_LAUNCHED();
     }
     k_fill0<<<(3 * n + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->adj_pt, 3 * n);
     CAGR_LAUNCHED();
-    const int64_t blocks = ((n + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t blocks = (n + 63) / 64;   // one wave per block
+    p.tile_order = t->tile_order;
     if (precision == CAGR_PREC_FP32C) {
         auto nodes = (const NodeV32*)t->nodes;
         auto src = (const SrcV32*)t->src;
 #define CAGR_ADJ32(GMA, CT)                                                                      \
     do {                                                                                         \
         if (slots)                                                                               \
-            k_adj32<GMA, CT, FM_SLOT><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src,       \
-                                                                   t->pts64, t->pts32);          \
+            k_adj32_list<GMA, CT><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src,                  \
+                                                               t->pts64, t->pts32);              \
         else                                                                                     \
-            k_adj32<GMA, CT, FM_ATOMIC><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src,     \
+            k_adj32<GMA, CT, FM_ATOMIC><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src,            \
                                                                      t->pts64, t->pts32);        \
     } while (0)
         if (ga) {
@@ -513,11 +650,11 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
         auto nodes = (const NodeV64*)t->nodes;
         auto src = (const SrcV64*)t->src;
         if (ga) {
-            if (continuous) k_adj64<1, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
-            else k_adj64<1, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+            if (continuous) k_adj64<1, true><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+            else k_adj64<1, false><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64);
         } else {
-            if (continuous) k_adj64<0, true><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
-            else k_adj64<0, false><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+            if (continuous) k_adj64<0, true><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64);
+            else k_adj64<0, false><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64);
         }
     }
     CAGR_LAUNCHED();
@@ -541,8 +678,15 @@ void adjoint_plan_free(cagr_tree* t) {
     (void)hipFree(t->plan_node_off);
     (void)hipFree(t->plan_slot_pos);
     (void)hipFree(t->plan_far_val);
+    (void)hipFree(t->plan_far_node);
+    (void)hipFree(t->plan_far_mask);
+    (void)hipFree(t->plan_near_off);
+    (void)hipFree(t->plan_near_leaf);
+    (void)hipFree(t->plan_near_mask);
     t->plan_w = nullptr;
     t->plan_wave_off = t->plan_node_off = t->plan_slot_pos = nullptr;
+    t->plan_far_node = t->plan_near_off = t->plan_near_leaf = nullptr;
+    t->plan_far_mask = t->plan_near_mask = nullptr;
     t->plan_far_val = nullptr;
     t->plan_slots = -1;
 }
@@ -556,11 +700,14 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
     // centres of these weights (the walk needs no moments)
     int rc = upward(t, 0, CAGR_PREC_FP32C, 1, 0, nullptr, CAGR_F64, t->plan_w, 0.0, st);
     if (rc) return rc;
-    const int64_t blocks = ((n + 63) / 64 + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int64_t nw = blocks * kWavesPerBlock;
+    const int64_t blocks = (n + 63) / 64;   // one wave per block, as the adjoint launches
+    const int64_t nw = blocks;
     CAGR_HIP(hipMalloc(&t->plan_wave_off, (nw + 1) * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&t->plan_near_off, (nw + 1) * sizeof(int32_t)));
     CAGR_HIP(hipMalloc(&t->plan_node_off, (m + 1) * sizeof(int32_t)));
     k_fill0_i32<<<(nw + 1 + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->plan_wave_off, nw + 1);
+    CAGR_LAUNCHED();
+    k_fill0_i32<<<(nw + 1 + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->plan_near_off, nw + 1);
     CAGR_LAUNCHED();
     k_fill0_i32<<<(m + 1 + kThreads - 1) / kThreads, kThreads, 0, st>>>(t->plan_node_off, m + 1);
     CAGR_LAUNCHED();
@@ -571,10 +718,12 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
     p.perm = t->perm;
     p.open_factor = t->open_factor;
     p.wave_cnt = t->plan_wave_off;
+    p.near_cnt = t->plan_near_off;
     p.node_cnt = t->plan_node_off;
     auto nodes = (const NodeV32*)t->nodes;
     auto src = (const SrcV32*)t->src;
-    k_adj32<0, false, FM_COUNT><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+    p.tile_order = t->tile_order;
+    k_adj32<0, false, FM_COUNT><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
     CAGR_LAUNCHED();
     // counts -> offsets
     int32_t* tmp = nullptr;
@@ -582,14 +731,20 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
     struct FreeTmp { int32_t** q; ~FreeTmp() { (void)hipFree(*q); } } free_tmp{&tmp};
     if ((rc = scan_exclusive<int32_t>(t->plan_wave_off, nw + 1, tmp, st))) return rc;
     if ((rc = scan_exclusive<int32_t>(t->plan_node_off, m + 1, tmp, st))) return rc;
-    int32_t slots32 = 0;
-    CAGR_HIP(hipMemcpyAsync(&slots32, t->plan_wave_off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if ((rc = scan_exclusive<int32_t>(t->plan_near_off, nw + 1, tmp, st))) return rc;
+    int32_t tot[2] = {0, 0};
+    CAGR_HIP(hipMemcpyAsync(&tot[0], t->plan_wave_off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipMemcpyAsync(&tot[1], t->plan_near_off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
-    const int64_t F = slots32;
-    CAGR_REQUIRE(F >= 0, CAGR_ERR_ARG, "adjoint plan: more than 2^31 far steps");
+    const int64_t F = tot[0], NN = tot[1];
+    CAGR_REQUIRE(F >= 0 && NN >= 0, CAGR_ERR_ARG, "adjoint plan: more than 2^31 steps");
     CAGR_HIP(hipMalloc(&t->plan_far_val, std::max<int64_t>(F, 1) * sizeof(float4)));
     CAGR_HIP(hipMalloc(&t->plan_slot_pos, std::max<int64_t>(F, 1) * sizeof(int32_t)));
-    if (F > 0) {
+    CAGR_HIP(hipMalloc(&t->plan_far_node, std::max<int64_t>(F, 1) * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&t->plan_far_mask, std::max<int64_t>(F, 1) * sizeof(uint64_t)));
+    CAGR_HIP(hipMalloc(&t->plan_near_leaf, std::max<int64_t>(NN, 1) * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&t->plan_near_mask, std::max<int64_t>(NN, 1) * sizeof(uint64_t)));
+    if (F > 0 || NN > 0) {
         // the node of every slot; the slots grouped by node (stable: by slot
         // id); the inverse of that order is each slot's node-major position
         uint64_t *keys = nullptr, *ktmp = nullptr;
@@ -612,13 +767,20 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
         p.wave_off = t->plan_wave_off;
         p.slot_node = keys;
         p.slot_id = ids;
-        k_adj32<0, false, FM_FILL><<<blocks, kThreads, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
+        p.far_node = t->plan_far_node;
+        p.far_mask = t->plan_far_mask;
+        p.near_off = t->plan_near_off;
+        p.near_leaf = t->plan_near_leaf;
+        p.near_mask = t->plan_near_mask;
+        k_adj32<0, false, FM_FILL><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64, t->pts32);
         CAGR_LAUNCHED();
         int bits = 1;
         while ((int64_t(1) << bits) < m) ++bits;
-        if ((rc = sort_pairs_u64(keys, ids, ktmp, vtmp, hist, F, bits, st))) return rc;
-        k_invert<<<(F + kThreads - 1) / kThreads, kThreads, 0, st>>>(ids, F, t->plan_slot_pos);
-        CAGR_LAUNCHED();
+        if (F > 0) {
+            if ((rc = sort_pairs_u64(keys, ids, ktmp, vtmp, hist, F, bits, st))) return rc;
+            k_invert<<<(F + kThreads - 1) / kThreads, kThreads, 0, st>>>(ids, F, t->plan_slot_pos);
+            CAGR_LAUNCHED();
+        }
         CAGR_HIP(hipStreamSynchronize(st));
     }
     t->plan_slots = F;

@@ -69,9 +69,10 @@ int check_common(const cagr_tree* t, int op, int precision, int dtype, OpInfo* o
 }
 
 int run_upward_for(cagr_tree* t, int op, int precision, const OpInfo& oi, const void* attrs,
-                   int dtype, const void* weights, double an, hipStream_t st) {
+                   int dtype, const void* weights, double an, hipStream_t st, int wdtype = -1) {
     const double an_rec = oi.convects ? an : 0.0;
-    return upward(t, oi.family, precision, oi.K, oi.dir0, attrs, dtype, weights, an_rec, st);
+    return upward(t, oi.family, precision, oi.K, oi.dir0, attrs, dtype, weights, an_rec, st,
+                  wdtype);
 }
 
 int apply_impl(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
@@ -203,8 +204,11 @@ int cagr_set_launch(cagr_tree* t, int persistent, int waves_per_block, int64_t g
     CAGR_REQUIRE(waves_per_block >= 1 && waves_per_block <= 8, CAGR_ERR_ARG,
                  "waves_per_block must be in [1, 8]");
     CAGR_REQUIRE(grid_blocks >= 0, CAGR_ERR_ARG, "grid_blocks must be >= 0");
+    CAGR_REQUIRE(persistent || waves_per_block <= 4, CAGR_ERR_ARG,
+                 "the regular-grid traversal takes at most 4 waves per block");
     t->persistent = persistent != 0;
-    t->waves_per_block = waves_per_block;
+    if (persistent) t->waves_per_block = waves_per_block;
+    else t->grid_waves = waves_per_block;
     t->grid_blocks = grid_blocks;
     return CAGR_OK;
 }
@@ -215,6 +219,8 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
     if (!query_cost) {   // back to natural order, equal contiguous chunks
         (void)hipFree(t->item_order);
         t->item_order = nullptr;
+        (void)hipFree(t->tile_order);
+        t->tile_order = nullptr;
         for (int c = 0; c <= kQueues; ++c) t->item_bound[c] = (int32_t)(t->num_items * c / kQueues);
         return CAGR_OK;
     }
@@ -247,6 +253,26 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
     CAGR_HIP(hipMemcpy(t->item_order, order.data(), t->num_items * sizeof(int32_t),
                        hipMemcpyHostToDevice));
     for (int c = 0; c <= kQueues; ++c) t->item_bound[c] = bound[c];
+    if (t->num_clouds == 1) {
+        // the regular grid (one wave per block, block b on XCD b mod 8, the
+        // XCD's blocks covering one contiguous range of dispatch slots): each
+        // XCD's range of waves, longest first
+        const int64_t nw = (t->n + 63) / 64;
+        std::vector<int64_t> wcost(nw, 0);
+        for (int64_t s = 0; s < t->n; ++s) wcost[s / 64] += query_cost[perm[s]];
+        std::vector<int32_t> torder(nw);
+        const int64_t q = nw / kQueues, r = nw % kQueues;
+        int64_t lo = 0;
+        for (int x = 0; x < kQueues; ++x) {
+            const int64_t len = q + (x < r ? 1 : 0);
+            for (int64_t j = lo; j < lo + len; ++j) torder[j] = (int32_t)j;
+            std::stable_sort(torder.begin() + lo, torder.begin() + lo + len,
+                             [&](int32_t a, int32_t b) { return wcost[a] > wcost[b]; });
+            lo += len;
+        }
+        if (!t->tile_order) CAGR_HIP(hipMalloc(&t->tile_order, std::max<int64_t>(nw, 1) * sizeof(int32_t)));
+        CAGR_HIP(hipMemcpy(t->tile_order, torder.data(), nw * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     return CAGR_OK;
 }
 
@@ -271,6 +297,7 @@ int cagr_tree_destroy(cagr_tree* t) {
     (void)hipFree(t->lbound);
     (void)hipFree(t->items);
     (void)hipFree(t->item_order);
+    (void)hipFree(t->tile_order);
     (void)hipFree(t->item_counter);
     (void)hipFree(t->adj_node);
     (void)hipFree(t->adj_pt);

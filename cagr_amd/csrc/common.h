@@ -120,9 +120,20 @@ struct cagr_tree {
     int32_t* plan_node_off = nullptr;  // [m + 1]
     int32_t* plan_slot_pos = nullptr;  // [slots] node-major position of each slot
     float4* plan_far_val = nullptr;    // [slots] node-major
+    // the recorded walk: far steps (node, lanes) wave-major in slot order, and
+    // near leaves (leaf, lanes) per wave from near_off
+    int32_t* plan_far_node = nullptr;  // [slots]
+    uint64_t* plan_far_mask = nullptr; // [slots]
+    int32_t* plan_near_off = nullptr;  // [waves + 1]
+    int32_t* plan_near_leaf = nullptr; // [near entries]
+    uint64_t* plan_near_mask = nullptr;// [near entries]
     int64_t plan_slots = -1;           // -1: no plan
     int persistent = 0;              // 1: traversal through the item queue
-    int waves_per_block = 4;         // launch shape of the traversal kernels
+    int waves_per_block = 4;         // launch shape of the persistent traversal kernels
+    int grid_waves = 1;              // waves per block of the regular-grid traversal kernels
+    // dispatch order of the regular grid's waves (one wave per block; null:
+    // Morton order): each XCD's contiguous range of waves, longest first
+    int32_t* tile_order = nullptr;
     int num_cus = 256;               // compute units of the device
     int64_t grid_blocks = 0;         // persistent grid size (0 = from the device)
 
@@ -151,7 +162,7 @@ template <typename U>
 int scan_exclusive(U* a, int64_t len, U* tmp, hipStream_t st);
 // implemented in upward.hip
 int upward(cagr_tree* t, int family, int precision, int K, int dir0, const void* attrs,
-           int dtype, const void* weights, double an, hipStream_t st);
+           int dtype, const void* weights, double an, hipStream_t st, int wdtype = -1);
 // implemented in traverse.hip
 struct Queries {
     int mode = 0;                 // 0 self, 1 explicit, 2 grid

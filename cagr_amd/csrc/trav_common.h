@@ -105,6 +105,13 @@ __device__ inline const T& rec32(const T* base, int32_t i) {
     return *(const T*)((const char*)base + (uint64_t)((uint32_t)i * (uint32_t)sizeof(T)));
 }
 
+// lane `lane`'s 64-bit value to every lane (a scalar)
+__device__ inline uint64_t lane_bcast_u64(uint64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <typename T>
 __device__ inline T lane_bcast(T v, int lane) {
     if constexpr (sizeof(T) == 4) {

@@ -69,6 +69,7 @@ struct TP {
     int64_t* cnt_visit;
     double open_factor;       // 2 / theta
     const float4* lbound;     // leaf bounding spheres
+    const int32_t* tile_order;// regular grid: dispatch slot -> wave (null: identity)
 };
 
 using namespace dev;
@@ -193,6 +194,36 @@ struct VAcc64 {
     }
 };
 
+// outputs of one query of the vector family from its FP32c sums
+template <int VOP, bool CONT>
+__device__ inline void finish_v32(const TP& p, const VAcc32<VOP, CONT>& acc, int64_t oi,
+                                  const double* x, double w) {
+    double res[3];
+    for (int k = 0; k < acc.NO; ++k) res[k] = acc.s[k];
+    if constexpr (VOP == VOP_A) {
+        double v = res[0];
+        if constexpr (CONT) v += (double)acc.P / (w * w * w);
+        store_out(p, oi, p.sign * v);
+    } else if constexpr (VOP == VOP_G) {
+        for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, res[k]);
+    } else {
+        double o[3];
+        const double ew = CONT ? exp(p.an * w) : 0.0;
+        for (int k = 0; k < 3; ++k) {
+            o[k] = exp(0.5 * p.an * x[k]) * res[k];
+            if constexpr (CONT)
+                o[k] += ew * ((double)acc.P / (w * w * w) + 0.5 * p.an * (double)acc.Q[k] / w);
+        }
+        if constexpr (VOP == VOP_A3) {
+            store_out(p, oi, ((o[0] + o[1]) + o[2]) / 3.0);
+        } else if (p.kdir >= 0) {
+            store_out(p, oi, sel3(o, p.kdir));
+        } else {
+            for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, o[k]);
+        }
+    }
+}
+
 // one wave: queries idx0 + lane (< qend), tree nodes [root, stop)
 template <int P, int VOP, int QM, bool CONT, bool CNT>
 __device__ __forceinline__ void trav_v(
@@ -292,32 +323,7 @@ __device__ __forceinline__ void trav_v(
         }
         if constexpr (!CNT) {
             acc.flush_leaf();
-            if (valid) {
-                double res[3];
-                for (int k = 0; k < acc.NO; ++k) res[k] = acc.s[k];
-                if constexpr (VOP == VOP_A) {
-                    double v = res[0];
-                    if constexpr (CONT) v += (double)acc.P / (w * w * w);
-                    store_out(p, oi, p.sign * v);
-                } else if constexpr (VOP == VOP_G) {
-                    for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, res[k]);
-                } else {
-                    double o[3];
-                    const double ew = CONT ? exp(p.an * w) : 0.0;
-                    for (int k = 0; k < 3; ++k) {
-                        o[k] = exp(0.5 * p.an * x[k]) * res[k];
-                        if constexpr (CONT)
-                            o[k] += ew * ((double)acc.P / (w * w * w) + 0.5 * p.an * (double)acc.Q[k] / w);
-                    }
-                    if constexpr (VOP == VOP_A3) {
-                        store_out(p, oi, ((o[0] + o[1]) + o[2]) / 3.0);
-                    } else if (p.kdir >= 0) {
-                        store_out(p, oi, sel3(o, p.kdir));
-                    } else {
-                        for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, o[k]);
-                    }
-                }
-            }
+            if (valid) finish_v32<VOP, CONT>(p, acc, oi, x, w);
         }
     } else {
         const NodeV64* __restrict__ nodes = (const NodeV64*)nodes_;
@@ -625,7 +631,8 @@ __global__ __launch_bounds__(kThreads) void k_trav_v(
     TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
     const float* __restrict__ pts32) {
-    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (p.tile_order) wave = p.tile_order[wave];
     trav_v<P, VOP, QM, CONT, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root,
                                   p.stop);
 }
@@ -635,7 +642,8 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
     TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
     const float* __restrict__ pts32) {
-    const int64_t wave = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (p.tile_order) wave = p.tile_order[wave];
     trav_s<P, K, QM, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root, p.stop);
 }
 
@@ -702,11 +710,11 @@ __global__ void k_reset_counter(int32_t* c) {
 template <int P, int VOP, int QM, bool CNT>
 int launch_v(const TP& tp, cagr_tree* t, int continuous, int64_t blocks, hipStream_t st) {
     if (continuous)
-        k_trav_v<P, VOP, QM, true, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src,
-                                                                    t->pts64, t->pts32);
+        k_trav_v<P, VOP, QM, true, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(
+            tp, t->topo, t->nodes, t->src, t->pts64, t->pts32);
     else
-        k_trav_v<P, VOP, QM, false, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src,
-                                                                     t->pts64, t->pts32);
+        k_trav_v<P, VOP, QM, false, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(
+            tp, t->topo, t->nodes, t->src, t->pts64, t->pts32);
     CAGR_LAUNCHED();
     return CAGR_OK;
 }
@@ -724,8 +732,8 @@ int launch_vop(const TP& tp, cagr_tree* t, int vop, int continuous, int64_t bloc
 
 template <int P, int K, int QM, bool CNT>
 int launch_s(const TP& tp, cagr_tree* t, int64_t blocks, hipStream_t st) {
-    k_trav_s<P, K, QM, CNT><<<blocks, kThreads, 0, st>>>(tp, t->topo, t->nodes, t->src, t->pts64,
-                                                         t->pts32);
+    k_trav_s<P, K, QM, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes, t->src,
+                                                                   t->pts64, t->pts32);
     CAGR_LAUNCHED();
     return CAGR_OK;
 }
@@ -856,7 +864,7 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     tp.nq = lanes;
     if (lanes == 0) return CAGR_OK;
     const int64_t waves = (lanes + 63) / 64;
-    const int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t blocks = (waves + t->grid_waves - 1) / t->grid_waves;
     const bool cnt = cnt_far || cnt_near || cnt_visit;
     const bool p64 = precision == CAGR_PREC_FP64;
     if (q.mode == QM_SELF && (t->persistent || t->num_clouds > 1)) {
@@ -872,6 +880,7 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     else if (p64) rc = launch_all<CAGR_PREC_FP64, QMV, false>(tp, t, op, continuous, blocks, st); \
     else rc = launch_all<CAGR_PREC_FP32C, QMV, false>(tp, t, op, continuous, blocks, st);
     if (q.mode == QM_SELF) {
+        if (t->grid_waves == 1 && !cnt) tp.tile_order = t->tile_order;
         CAGR_LAUNCH_QM(QM_SELF)
     } else if (q.mode == QM_EXPLICIT) {
         CAGR_LAUNCH_QM(QM_EXPLICIT)

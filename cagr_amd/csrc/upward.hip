@@ -29,6 +29,11 @@ inline int64_t blocks_for(int64_t n) { return (n + kThreads - 1) / kThreads; }
 template <typename TI>
 __device__ inline double ld64(const TI* p, int64_t i) { return p ? (double)p[i] : 0.0; }
 
+// point weights in their own dtype (they may differ from the attributes')
+__device__ inline double wload(const void* w, int wdtype, int64_t i) {
+    return wdtype == CAGR_F32 ? (double)((const float*)w)[i] : ((const double*)w)[i];
+}
+
 // record writers --------------------------------------------------------------
 template <int FAM, int K, int P>
 struct Rec;
@@ -130,7 +135,7 @@ template <typename TI, int FAM, int K, int P>
 __global__ __launch_bounds__(kThreads) void k_up_leaf(
     const int32_t* __restrict__ order, int64_t cnt, const TopoRec* __restrict__ topo,
     const int32_t* __restrict__ perm, const double* __restrict__ pts64,
-    const TI* __restrict__ attrs, const TI* __restrict__ weights, int64_t m,
+    const TI* __restrict__ attrs, const void* __restrict__ weights, int wdtype, int64_t m,
     double* __restrict__ w64, double* __restrict__ r64, double* __restrict__ m64,
     void* __restrict__ nodes, double an, int dir0) {
 #pragma clang fp contract(off)
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(kThreads) void k_up_leaf(
             for (int c = 0; c < C; ++c) a[c] = ld64(attrs, j * (K * C) + k * C + c);
             double w;
             if (weights) {
-                w = ld64(weights, j);
+                w = wload(weights, wdtype, j);
             } else {
                 double ss = a[0] * a[0];
                 for (int c = 1; c < C; ++c) ss = ss + a[c] * a[c];
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_up_internal(
 }
 
 template <typename TI, int FAM, int K, int P>
-int run_upward(cagr_tree* t, const TI* attrs, const TI* weights, double an, int dir0,
+int run_upward(cagr_tree* t, const TI* attrs, const void* weights, int wdtype, double an, int dir0,
                hipStream_t st) {
     const int64_t n = t->n, m = t->m;
     if (m == 0) return CAGR_OK;
@@ -243,7 +248,7 @@ int run_upward(cagr_tree* t, const TI* attrs, const TI* weights, double an, int 
     const auto& g = t->group_off;
     const int64_t nleaf = g[1] - g[0];
     k_up_leaf<TI, FAM, K, P><<<blocks_for(nleaf), kThreads, 0, st>>>(
-        t->order + g[0], nleaf, t->topo, t->perm, t->pts64, attrs, weights, m, t->w64, t->r64,
+        t->order + g[0], nleaf, t->topo, t->perm, t->pts64, attrs, weights, wdtype, m, t->w64, t->r64,
         t->m64, t->nodes, an, dir0);
     CAGR_LAUNCHED();
     for (size_t gi = 1; gi + 1 < g.size(); ++gi) {
@@ -258,30 +263,32 @@ int run_upward(cagr_tree* t, const TI* attrs, const TI* weights, double an, int 
 
 template <typename TI>
 int dispatch_upward(cagr_tree* t, int family, int precision, int K, int dir0, const TI* attrs,
-                    const TI* weights, double an, hipStream_t st) {
+                    const void* weights, int wdtype, double an, hipStream_t st) {
     const bool p64 = precision == CAGR_PREC_FP64;
     if (family == 0) {
-        return p64 ? run_upward<TI, 0, 1, CAGR_PREC_FP64>(t, attrs, weights, an, dir0, st)
-                   : run_upward<TI, 0, 1, CAGR_PREC_FP32C>(t, attrs, weights, an, dir0, st);
+        return p64 ? run_upward<TI, 0, 1, CAGR_PREC_FP64>(t, attrs, weights, wdtype, an, dir0, st)
+                   : run_upward<TI, 0, 1, CAGR_PREC_FP32C>(t, attrs, weights, wdtype, an, dir0, st);
     }
     if (K == 1)
-        return p64 ? run_upward<TI, 1, 1, CAGR_PREC_FP64>(t, attrs, weights, an, dir0, st)
-                   : run_upward<TI, 1, 1, CAGR_PREC_FP32C>(t, attrs, weights, an, dir0, st);
-    return p64 ? run_upward<TI, 1, 3, CAGR_PREC_FP64>(t, attrs, weights, an, dir0, st)
-               : run_upward<TI, 1, 3, CAGR_PREC_FP32C>(t, attrs, weights, an, dir0, st);
+        return p64 ? run_upward<TI, 1, 1, CAGR_PREC_FP64>(t, attrs, weights, wdtype, an, dir0, st)
+                   : run_upward<TI, 1, 1, CAGR_PREC_FP32C>(t, attrs, weights, wdtype, an, dir0, st);
+    return p64 ? run_upward<TI, 1, 3, CAGR_PREC_FP64>(t, attrs, weights, wdtype, an, dir0, st)
+               : run_upward<TI, 1, 3, CAGR_PREC_FP32C>(t, attrs, weights, wdtype, an, dir0, st);
 }
 
 }  // namespace
 
-// family 0 = vector (attrs [n,3]), 1 = scalar sets (attrs [n,K]).
+// family 0 = vector (attrs [n,3]), 1 = scalar sets (attrs [n,K]); weights in
+// wdtype (-1: the attributes' dtype)
 int upward(cagr_tree* t, int family, int precision, int K, int dir0, const void* attrs,
-           int dtype, const void* weights, double an, hipStream_t st) {
+           int dtype, const void* weights, double an, hipStream_t st, int wdtype) {
     CAGR_REQUIRE(family == 0 ? K == 1 : (K == 1 || K == 3), CAGR_ERR_ARG, "bad upward set count");
+    if (wdtype < 0) wdtype = dtype;
     if (dtype == CAGR_F32)
-        return dispatch_upward<float>(t, family, precision, K, dir0, (const float*)attrs,
-                                      (const float*)weights, an, st);
-    return dispatch_upward<double>(t, family, precision, K, dir0, (const double*)attrs,
-                                   (const double*)weights, an, st);
+        return dispatch_upward<float>(t, family, precision, K, dir0, (const float*)attrs, weights,
+                                      wdtype, an, st);
+    return dispatch_upward<double>(t, family, precision, K, dir0, (const double*)attrs, weights,
+                                   wdtype, an, st);
 }
 
 }  // namespace cagr

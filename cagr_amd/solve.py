@@ -83,8 +83,8 @@ def cagr_schedule(wt: WindingNumberTreecode, wsmin: float, wsmax: float, iters: 
 class NormalOperator:
     """A mu = -[A21 mu; A22 mu; A23 mu] (as [N,3]) with node centres fixed by
     unit weights, and its exact adjoint; counts operator applications.  With
-    ``plan`` the FP32c adjoint runs through the tree's adjoint plan (built
-    here, once: the centres never change during a solve)."""
+    ``plan`` the FP32c adjoint replays the walk recorded by the tree's adjoint
+    plan (built here, once: the centres never change during a solve)."""
 
     def __init__(self, wt: WindingNumberTreecode, width: float, an: float = 1.0,
                  plan: bool = False):

@@ -191,6 +191,23 @@ class WindingNumberTreecode:
                                             _ptr(out), _stream(self.device)))
         return out
 
+    def balance(self, op: str = "A2", attrs: Optional[torch.Tensor] = None) -> np.ndarray:
+        """Dispatch the traversal's waves longest first (within each XCD's part
+        of the tree; the batched form's work items likewise), costed by the
+        exact interaction counts of `op` on `attrs` (default: unit
+        attributes).  Returns the per-query cost used."""
+        if attrs is None:
+            attrs = torch.ones((self.points.shape[0], IN_DIM[op]), dtype=self.points.dtype,
+                               device=self.device)
+        far, near, _ = self.counts(op, attrs)
+        cost = (far + near).cpu().numpy().astype(np.int64)
+        check(self.L.cagr_order_items(self._h, cost.ctypes.data_as(ct.c_void_p)))
+        return cost
+
+    def unbalance(self) -> None:
+        """Back to Morton dispatch order."""
+        check(self.L.cagr_order_items(self._h, None))
+
     def adjoint_plan(self, weights: torch.Tensor) -> None:
         """Record the far-field step sequence of the FP32 adjoint for the node
         centres of ``weights`` [N] (an iterative solve applies the adjoint many

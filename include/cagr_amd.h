@@ -183,9 +183,9 @@ int cagr_forest_build(const void* points, const int64_t* cloud_offsets, int64_t 
 /* Node id ranges of the clouds: node_offsets [num_clouds + 1] (HOST). */
 int cagr_forest_roots(const cagr_tree* tree, int64_t* node_offsets);
 /* Launch shape of the traversal: persistent = 1 routes single-cloud trees
- * through the persistent work-queue kernel too (forests always use it);
- * waves_per_block in [1, 8]; grid_blocks = 0 sizes the persistent grid to 8
- * waves per SIMD on every CU. */
+ * through the persistent work-queue kernel too (forests always use it) with
+ * waves_per_block in [1, 8] and grid_blocks (0: 8 waves per SIMD on every
+ * CU); persistent = 0 sets the regular grid's waves per block, in [1, 4]. */
 int cagr_set_launch(cagr_tree* tree, int persistent, int waves_per_block, int64_t grid_blocks);
 /* Longest-first order of the persistent work items from a per-query cost
  * (HOST int64 [n], original order, e.g. n_far + n_near from cagr_count);

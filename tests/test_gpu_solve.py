@@ -120,6 +120,28 @@ def test_planned_adjoint_matches_atomic_and_is_deterministic(op):
     assert rel(a, r64) <= 2e-6
 
 
+@pytest.mark.parametrize("op,cont", [("A2", False), ("A3", True), ("A", False)])
+def test_planned_adjoint_dot_product_identity(op, cont):
+    # <A mu, g> = <mu, A^T g>: the forward with the plan's weights against the
+    # planned adjoint
+    say("planned adjoint dot product", op, cont)
+    pts = cloud(30000, 8)
+    n = pts.shape[0]
+    wt = WindingNumberTreecode(pts, leaf_size=32)
+    gen = torch.Generator(device="cpu").manual_seed(13)
+    w1 = torch.ones(n, dtype=torch.float64, device=DEV)
+    widths = torch.full((n,), 0.01, dtype=torch.float64, device=DEV)
+    mu = torch.randn(n, 3, generator=gen, dtype=torch.float64).to(DEV)
+    wt.adjoint_plan(w1)
+    a = wt.apply(op, mu, widths, weights=w1, continuous=cont, precision="fp64")
+    g = torch.randn(a.shape[0], a.shape[1], generator=gen, dtype=torch.float64).to(DEV)
+    at = wt.apply_adjoint_planned(op, g, widths, continuous=cont, precision="fp32c")
+    lhs = float((a * g).sum())
+    rhs = float((mu * at).sum())
+    scale = float(a.abs().sum() * g.abs().max())
+    assert abs(lhs - rhs) <= 2e-6 * scale
+
+
 def test_planned_adjoint_weights_and_rebuild():
     say("planned adjoint weights")
     pts = cloud(8000, 9)
