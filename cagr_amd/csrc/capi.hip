@@ -287,6 +287,7 @@ int cagr_tree_destroy(cagr_tree* t) {
     (void)hipFree(t->keys);
     (void)hipFree(t->topo);
     (void)hipFree(t->order);
+    (void)hipFree(t->up_seg);
     (void)hipFree(t->src);
     (void)hipFree(t->nodes);
     (void)hipFree(t->w64);

@@ -288,27 +288,91 @@ __global__ __launch_bounds__(kThreads) void k_leaf_bounds(const TopoRec* __restr
     lb[i] = make_float4(c[0], c[1], c[2], (r2 == r2 && r2 < 1e300) ? r : 3.0e38f);
 }
 
-// group of a node in the upward order: 0 = leaf, 1 + (14 - depth) = internal
-__device__ inline int up_group(const TopoRec& t) {
-    return topo_leaf(t) ? 0 : 1 + (kLevels - 1 - topo_depth(t));
-}
-constexpr int kGroups = 1 + kLevels;
-
-__global__ __launch_bounds__(kThreads) void k_group_count(const TopoRec* __restrict__ topo,
-                                                          int64_t m, int32_t* __restrict__ cnt,
-                                                          int32_t* __restrict__ max_depth) {
+// nodes keyed by depth for a stable sort (preorder within a depth); leaf
+// count and deepest leaf, aggregated per block
+__global__ __launch_bounds__(kThreads) void k_depth_keys(const TopoRec* __restrict__ topo, int64_t m,
+                                                         uint64_t* __restrict__ keys,
+                                                         int32_t* __restrict__ vals,
+                                                         int32_t* __restrict__ stats) {
+    __shared__ int32_t s_leaves, s_depth;
+    if (threadIdx.x == 0) { s_leaves = 0; s_depth = 0; }
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < m) {
+        const TopoRec t = topo[i];
+        keys[i] = (uint64_t)topo_depth(t);
+        vals[i] = (int32_t)i;
+        if (topo_leaf(t)) {
+            atomicAdd(&s_leaves, 1);
+            atomicMax(&s_depth, topo_depth(t));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&stats[0], s_leaves);
+        atomicMax(&stats[1], s_depth);
+    }
+}
+
+// first index of each depth in the depth-sorted keys: off[d], d in
+// [0, kLevels + 1] (off[kLevels + 1] = m, the end of the deepest depth)
+__global__ void k_depth_off(const uint64_t* __restrict__ keys, int64_t m, int32_t* __restrict__ off) {
+    const int d = threadIdx.x;
+    if (d > kLevels + 1) return;
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint64_t)d) lo = mid + 1;
+        else hi = mid;
+    }
+    off[d] = (int32_t)lo;
+}
+
+// First child's position in the depth order and child count of every
+// internal node, from its first child (positions of depth >= 1).  The
+// children of p are consecutive there: within a depth the order is ascending
+// id, and a node whose id lies between two children of p is inside p's
+// subtree, so at p's depth + 1 it is a child of p too.
+__global__ __launch_bounds__(kThreads) void k_kids(TopoRec* __restrict__ topo,
+                                                   const int32_t* __restrict__ order, int32_t i0,
+                                                   int64_t m) {
+    const int64_t i = i0 + (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= m) return;
-    const TopoRec t = topo[i];
-    atomicAdd(&cnt[up_group(t)], 1);
-    if (topo_leaf(t)) atomicMax(max_depth, topo_depth(t));
+    const int32_t p = topo[order[i]].parent;
+    if (topo[order[i - 1]].parent == p) return;  // not the first child
+    int64_t j = i + 1;
+    while (j < m && topo[order[j]].parent == p) ++j;
+    topo[p].kids = (int32_t)i;
+    topo[p].info = topo[p].info | ((int32_t)(j - i) << 12);
 }
 
-__global__ __launch_bounds__(kThreads) void k_group_fill(const TopoRec* __restrict__ topo,
-                                                         int64_t m, int32_t* __restrict__ cursor,
-                                                         int32_t* __restrict__ order) {
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i < m) order[atomicAdd(&cursor[up_group(topo[i])], 1)] = (int32_t)i;
+// Segments of the upward launches: for root i (the id-sorted list of roots
+// at depth dlo) and level l (depth dlo + l), the run of the depth-sorted
+// order holding the root's subtree nodes at that depth.
+__global__ __launch_bounds__(kThreads) void k_up_segments(const TopoRec* __restrict__ topo,
+                                                          const int32_t* __restrict__ order,
+                                                          const int32_t* __restrict__ off,
+                                                          int32_t nroots, int dlo, int nlev,
+                                                          int2* __restrict__ seg) {
+    const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (k >= (int64_t)nroots * nlev) return;
+    const int32_t i = (int32_t)(k / nlev);
+    const int l = (int)(k % nlev);
+    const int32_t r = order[off[dlo] + i];
+    const int32_t e = topo[r].escape;
+    const int d = dlo + l;
+    const int32_t* lst = order + off[d];
+    const int32_t len = off[d + 1] - off[d];
+    auto lower = [&](int32_t v) {
+        int32_t lo = 0, hi = len;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (lst[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    seg[k] = make_int2(off[d] + lower(r), off[d] + lower(e));
 }
 
 }  // namespace
@@ -422,17 +486,17 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     t->cloud_node.assign(t->num_clouds + 1, 0);
     if (n == 0) {   // the reference builds no node at all for an empty cloud
         t->m = 0;
-        t->group_off.assign(kGroups + 1, 0);
+        t->depth_off.assign(kLevels + 2, 0);
         return CAGR_OK;
     }
 
     // scratch for the build (freed at the end)
     uint64_t *ktmp = nullptr;
-    int32_t *vtmp = nullptr, *cnt = nullptr, *gcnt = nullptr;
+    int32_t *vtmp = nullptr, *cnt = nullptr;
     uint32_t* hist = nullptr;
     int8_t *ld = nullptr, *sd = nullptr;
     auto free_scratch = [&]() {
-        (void)hipFree(ktmp); (void)hipFree(vtmp); (void)hipFree(cnt); (void)hipFree(gcnt); (void)hipFree(hist);
+        (void)hipFree(ktmp); (void)hipFree(vtmp); (void)hipFree(cnt); (void)hipFree(hist);
         (void)hipFree(ld); (void)hipFree(sd);
     };
     struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{free_scratch};
@@ -447,7 +511,6 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     CAGR_HIP(hipMalloc(&cnt, (n + 1) * sizeof(int32_t)));
     CAGR_HIP(hipMalloc(&ld, n));
     CAGR_HIP(hipMalloc(&sd, n));
-    CAGR_HIP(hipMalloc(&gcnt, (2 * kGroups + 1) * sizeof(int32_t)));
 
     int64_t* coff_d = nullptr;
     struct Guard2 { int64_t** p; ~Guard2() { (void)hipFree(*p); } } guard2{&coff_d};
@@ -510,23 +573,96 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
         k_gather_points<double><<<nb, kThreads, 0, st>>>((const double*)points, t->perm, n, t->pts64, t->pts32);
     CAGR_LAUNCHED();
 
-    // upward order: all leaves, then internal nodes from depth 14 up to 0
-    CAGR_HIP(hipMemsetAsync(gcnt, 0, (2 * kGroups + 1) * sizeof(int32_t), st));
-    k_group_count<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, gcnt, gcnt + 2 * kGroups);
-    CAGR_LAUNCHED();
-    int32_t hc[kGroups], hdepth = 0;
-    CAGR_HIP(hipMemcpyAsync(hc, gcnt, sizeof(hc), hipMemcpyDeviceToHost, st));
-    CAGR_HIP(hipMemcpyAsync(&hdepth, gcnt + 2 * kGroups, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    CAGR_HIP(hipStreamSynchronize(st));
-    t->group_off.assign(kGroups + 1, 0);
-    for (int g = 0; g < kGroups; ++g) t->group_off[g + 1] = t->group_off[g] + hc[g];
-    int32_t cur[kGroups];
-    for (int g = 0; g < kGroups; ++g) cur[g] = (int32_t)t->group_off[g];
-    CAGR_HIP(hipMemcpyAsync(gcnt + kGroups, cur, sizeof(cur), hipMemcpyHostToDevice, st));
-    k_group_fill<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, gcnt + kGroups, t->order);
-    CAGR_LAUNCHED();
-    t->n_leaves = hc[0];
-    t->depth = hdepth;
+    // node ids by depth (stable: preorder within a depth), the depth offsets,
+    // and the segments of the two upward launches
+    {
+        uint64_t *dk = nullptr, *dkt = nullptr;
+        int32_t *dvt = nullptr, *off_d = nullptr, *stats = nullptr;
+        uint32_t* dh = nullptr;
+        struct FreeD {
+            uint64_t **a, **b;
+            int32_t **c, **d, **e;
+            uint32_t** f;
+            ~FreeD() {
+                (void)hipFree(*a); (void)hipFree(*b); (void)hipFree(*c); (void)hipFree(*d);
+                (void)hipFree(*e); (void)hipFree(*f);
+            }
+        } free_d{&dk, &dkt, &dvt, &off_d, &stats, &dh};
+        CAGR_HIP(hipMalloc(&dk, m * sizeof(uint64_t)));
+        CAGR_HIP(hipMalloc(&dkt, m * sizeof(uint64_t)));
+        CAGR_HIP(hipMalloc(&dvt, m * sizeof(int32_t)));
+        CAGR_HIP(hipMalloc(&dh, sort_hist_len(m) * sizeof(uint32_t)));
+        CAGR_HIP(hipMalloc(&off_d, (kLevels + 2) * sizeof(int32_t)));
+        CAGR_HIP(hipMalloc(&stats, 2 * sizeof(int32_t)));
+        CAGR_HIP(hipMemsetAsync(stats, 0, 2 * sizeof(int32_t), st));
+        k_depth_keys<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, dk, t->order, stats);
+        CAGR_LAUNCHED();
+        if ((rc = sort_pairs_u64(dk, t->order, dkt, dvt, dh, m, 4, st))) return rc;
+        k_depth_off<<<1, 64, 0, st>>>(dk, m, off_d);
+        CAGR_LAUNCHED();
+        int32_t hoff[kLevels + 2], hst[2];
+        CAGR_HIP(hipMemcpyAsync(hoff, off_d, sizeof(hoff), hipMemcpyDeviceToHost, st));
+        CAGR_HIP(hipMemcpyAsync(hst, stats, sizeof(hst), hipMemcpyDeviceToHost, st));
+        CAGR_HIP(hipStreamSynchronize(st));
+        t->depth_off.assign(hoff, hoff + kLevels + 2);
+        t->n_leaves = hst[0];
+        t->depth = hst[1];
+        // the segment kernel reads off[d] and off[d + 1] for every depth up to
+        // the deepest: they must be in order and end at m
+        bool ok = hoff[0] == 0 && hoff[kLevels + 1] == (int32_t)m && t->depth >= 0 &&
+                  t->depth <= kLevels;
+        for (int d = 0; d <= kLevels; ++d) ok = ok && hoff[d] <= hoff[d + 1];
+        CAGR_REQUIRE(ok, CAGR_ERR_STATE, "tree build: inconsistent depth offsets");
+        // the roots are at depth 0, so positions off[1].. have a parent and
+        // position off[1] - 1 >= 0 exists
+        if (m > hoff[1]) {
+            CAGR_REQUIRE(hoff[1] >= 1, CAGR_ERR_STATE, "tree build: no root");
+            k_kids<<<blocks_for(m - hoff[1]), kThreads, 0, st>>>(t->topo, t->order, hoff[1], m);
+            CAGR_LAUNCHED();
+        }
+        // deep launch: subtrees rooted at depth D (the first depth with >= 1024
+        // nodes, else the deepest), levels D..depth, up_group consecutive
+        // roots per wave (about 64 nodes), plus the leaves above depth D; then
+        // bands of at most 3 depths above D (at most 1 + 8 + 64 nodes under
+        // each band root), deepest first
+        const int dmax = t->depth;
+        auto count = [&](int d) { return t->depth_off[d + 1] - t->depth_off[d]; };
+        int D = 0;
+        if (dmax >= 1) {
+            D = dmax;
+            for (int d = 1; d <= dmax; ++d)
+                if (count(d) >= 1024) { D = d; break; }
+        }
+        t->up_D = D;
+        t->up_deep_roots = count(D);
+        t->up_shallow = t->depth_off[D];
+        const int64_t deep_nodes = (int64_t)m - t->up_shallow;
+        t->up_group = (int32_t)std::min<int64_t>(
+            64, std::max<int64_t>(1, 64 * (int64_t)t->up_deep_roots / std::max<int64_t>(1, deep_nodes)));
+        const int64_t nd = (int64_t)t->up_deep_roots * (dmax - D + 1);
+        t->up_bands.clear();
+        int64_t nt = nd;
+        for (int dhi = D; dhi > 0;) {
+            const int dlo = std::max(0, dhi - 3);
+            t->up_bands.push_back(make_int4(dlo, dhi - dlo, count(dlo), (int)nt));
+            nt += (int64_t)count(dlo) * (dhi - dlo);
+            dhi = dlo;
+        }
+        CAGR_REQUIRE(nt < (int64_t)INT32_MAX, CAGR_ERR_ARG, "tree build: too many upward segments");
+        CAGR_HIP(hipMalloc(&t->up_seg, std::max<int64_t>(nt, 1) * sizeof(int2)));
+        if (nd > 0) {
+            k_up_segments<<<blocks_for(nd), kThreads, 0, st>>>(t->topo, t->order, off_d,
+                                                               t->up_deep_roots, D, dmax - D + 1,
+                                                               t->up_seg);
+            CAGR_LAUNCHED();
+        }
+        for (const int4& b : t->up_bands) {
+            k_up_segments<<<blocks_for((int64_t)b.z * b.y), kThreads, 0, st>>>(
+                t->topo, t->order, off_d, b.z, b.x, b.y, t->up_seg + b.w);
+            CAGR_LAUNCHED();
+        }
+        CAGR_HIP(hipStreamSynchronize(st));
+    }
 
     // wave work items: per cloud, its node range and its points 64 at a time
     {

@@ -154,13 +154,14 @@ struct alignas(16) TopoRec {
     int32_t escape;   // preorder id of the first node after this subtree
     int32_t begin;    // first sorted point of the node
     int32_t end;      // one past its last sorted point
-    int32_t info;     // depth | (is_leaf << 8)
+    int32_t info;     // depth | (is_leaf << 8) | (child count << 12)
     int32_t parent;   // -1 for a root
-    int32_t pad;
+    int32_t kids;     // position of the first child in the depth order (-1: none)
 };
 
 CAGR_HD inline int topo_depth(const TopoRec& t) { return t.info & 0xff; }
 CAGR_HD inline bool topo_leaf(const TopoRec& t) { return (t.info >> 8) & 1; }
+CAGR_HD inline int topo_nkids(const TopoRec& t) { return (t.info >> 12) & 15; }
 
 }  // namespace cagr
 
@@ -202,7 +203,7 @@ CAGR_HD inline void node_body(int64_t b, const uint64_t* keys, int64_t n, const 
             const int64_t b2 = cell_begin(keys, b, d - 1);
             t.parent = off[b2] + (d - 1 - sd[b2]);
         }
-        t.pad = 0;
+        t.kids = -1;  // set once the depth order exists
         topo[id] = t;
     }
 }
