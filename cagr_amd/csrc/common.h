@@ -85,15 +85,11 @@ struct cagr_tree {
     int32_t* order = nullptr;     // [m] node ids by depth, preorder within a depth
     float4* lbound = nullptr;     // [m] bounding sphere {centre, radius} of each leaf's points
     std::vector<int32_t> depth_off;  // host: order[] offsets of depths 0..16 (size 17)
-    // the upward pass: one launch of waves over the subtrees rooted at depth
-    // up_D (levels up_D..depth, up_group roots per wave) and over the
-    // up_shallow nodes above (their leaves), then one launch per band of at
-    // most 3 depths above up_D, deepest band first, a wave per node at the
-    // band's top depth (the band's internal nodes); up_bands[b] = {top depth,
-    // depths, roots, first up_seg entry}; up_seg holds each root's run of
-    // order[] per level (the deep roots' first)
-    int up_D = 0;
-    int32_t up_deep_roots = 0, up_group = 1, up_shallow = 0;
+    // the upward pass: one launch over the leaves (64 consecutive node ids a
+    // wave), then one launch per band of at most 3 depths, deepest band
+    // first, a wave per node at the band's top depth (the band's internal
+    // nodes); up_bands[b] = {top depth, depths, roots, first up_seg entry};
+    // up_seg holds each band root's run of order[] per level
     std::vector<int4> up_bands;
     int2* up_seg = nullptr;
 

@@ -620,42 +620,21 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
             k_kids<<<blocks_for(m - hoff[1]), kThreads, 0, st>>>(t->topo, t->order, hoff[1], m);
             CAGR_LAUNCHED();
         }
-        // deep launch: subtrees rooted at depth D (the first depth with >= 1024
-        // nodes, else the deepest), levels D..depth, up_group consecutive
-        // roots per wave (about 64 nodes), plus the leaves above depth D; then
-        // bands of at most 3 depths above D (at most 1 + 8 + 64 nodes under
-        // each band root), deepest first
-        const int dmax = t->depth;
+        // bands of at most 3 depths over the internal nodes (depths
+        // 0..depth-1), deepest first: at most 1 + 8 + 64 nodes under each
+        // band root
         auto count = [&](int d) { return t->depth_off[d + 1] - t->depth_off[d]; };
-        int D = 0;
-        if (dmax >= 1) {
-            D = dmax;
-            for (int d = 1; d <= dmax; ++d)
-                if (count(d) >= 1024) { D = d; break; }
-        }
-        t->up_D = D;
-        t->up_deep_roots = count(D);
-        t->up_shallow = t->depth_off[D];
-        const int64_t deep_nodes = (int64_t)m - t->up_shallow;
-        t->up_group = (int32_t)std::min<int64_t>(
-            64, std::max<int64_t>(1, 64 * (int64_t)t->up_deep_roots / std::max<int64_t>(1, deep_nodes)));
-        const int64_t nd = (int64_t)t->up_deep_roots * (dmax - D + 1);
         t->up_bands.clear();
-        int64_t nt = nd;
-        for (int dhi = D; dhi > 0;) {
+        int64_t nt = 0;
+        for (int dhi = t->depth; dhi > 0;) {
             const int dlo = std::max(0, dhi - 3);
             t->up_bands.push_back(make_int4(dlo, dhi - dlo, count(dlo), (int)nt));
             nt += (int64_t)count(dlo) * (dhi - dlo);
+            CAGR_REQUIRE(nt < (int64_t)INT32_MAX, CAGR_ERR_ARG,
+                         "tree build: too many upward segments");
             dhi = dlo;
         }
-        CAGR_REQUIRE(nt < (int64_t)INT32_MAX, CAGR_ERR_ARG, "tree build: too many upward segments");
         CAGR_HIP(hipMalloc(&t->up_seg, std::max<int64_t>(nt, 1) * sizeof(int2)));
-        if (nd > 0) {
-            k_up_segments<<<blocks_for(nd), kThreads, 0, st>>>(t->topo, t->order, off_d,
-                                                               t->up_deep_roots, D, dmax - D + 1,
-                                                               t->up_seg);
-            CAGR_LAUNCHED();
-        }
         for (const int4& b : t->up_bands) {
             k_up_segments<<<blocks_for((int64_t)b.z * b.y), kThreads, 0, st>>>(
                 t->topo, t->order, off_d, b.z, b.x, b.y, t->up_seg + b.w);

@@ -12,12 +12,11 @@
 // passes fixed weights.  All FP64 moment arithmetic runs with contraction off
 // in that order, so the FP64 moments equal the reference CPU core's bit for
 // bit.  Instead of the reference's 1 + 2*tree_depth launches over all nodes
-// (find_next + nonleaf per level, cuda_iface:77-131), this runs about
-// 1 + D/3 launches: one wave per few subtrees rooted at a depth D (their
-// leaves from points staged through LDS, then their internal levels deepest
-// first, a barrier between levels) plus waves over the leaves above D, then a
-// launch per band of up to 3 depths above D, a wave per node at the band's
-// top.  The deep launch also writes the points' per-call source records.
+// (find_next + nonleaf per level, cuda_iface:77-131), this runs
+// 1 + ceil(depth/3) launches: one over the leaves (points staged through LDS,
+// the points' per-call source records written on the way), then one per band
+// of up to 3 depths, deepest first, a wave per node at the band's top depth
+// taking the band's internal nodes level by level.
 //
 // Several "sets" (K) of scalar attributes can share one pass (the three
 // inputs of the fused sum_k A2Tk): each set has its own weights and centres.
@@ -176,59 +175,6 @@ __device__ inline void leaf_finish(int32_t nd, const TopoRec& t, const double (&
     }
 }
 
-// Leaf node nd from its points (ascending sorted position = ascending
-// original index inside the leaf), writing the points' source records too
-// unless src is null.  Points are loaded U at a time (indices clamped into the
-// leaf, so every load is in bounds) and then summed one by one in order.
-template <typename TI, int FAM, int K, int P>
-__device__ inline void up_leaf_node(int32_t nd, const TopoRec& t, const int32_t* __restrict__ perm,
-                                    const double* __restrict__ pts64,
-                                    const float* __restrict__ pts32, const TI* __restrict__ attrs,
-                                    const void* __restrict__ weights, int wdtype, int64_t m,
-                                    double* __restrict__ w64, double* __restrict__ r64,
-                                    double* __restrict__ m64, void* __restrict__ nodes,
-                                    void* __restrict__ src, double an, int dir0) {
-#pragma clang fp contract(off)
-    constexpr int C = Rec<FAM, K, P>::C;
-    constexpr int U = 4;
-    double tw[K], rep[K][3], acc[K][C], rep0[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < K; ++k) {
-        tw[k] = 0.0;
-        for (int d = 0; d < 3; ++d) rep[k][d] = 0.0;
-        for (int c = 0; c < C; ++c) acc[k][c] = 0.0;
-    }
-    for (int32_t s0 = t.begin; s0 < t.end; s0 += U) {
-        PtIn<K * C, P> q[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int32_t s = min(s0 + u, t.end - 1);
-            load_point<TI, K * C, P>(q[u], s, (int64_t)perm[s], pts64, pts32, attrs, weights,
-                                     wdtype);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (s0 + u >= t.end) break;
-            if (src) Rec<FAM, K, P>::src(src, s0 + u, q[u].y, q[u].y32, q[u].a, an, dir0);
-            for (int k = 0; k < K; ++k) {
-                const double* a = q[u].a + k * C;
-                double w;
-                if (weights) {
-                    w = q[u].w;
-                } else {
-                    double ss = a[0] * a[0];
-                    for (int c = 1; c < C; ++c) ss = ss + a[c] * a[c];
-                    w = sqrt(ss);
-                }
-                tw[k] += w;
-                for (int c = 0; c < C; ++c) acc[k][c] += a[c];
-                for (int d = 0; d < 3; ++d) rep[k][d] += q[u].y[d] * w;
-            }
-            for (int d = 0; d < 3; ++d) rep0[d] += q[u].y[d];
-        }
-    }
-    leaf_finish<FAM, K, P>(nd, t, tw, rep, acc, rep0, m, w64, r64, m64, nodes, an, dir0);
-}
-
 // child ids of internal node t in octant order (ascending id): positions
 // kids.. of the depth order; entries past the child count repeat the last
 // child (in bounds, never summed)
@@ -288,151 +234,100 @@ __device__ inline void up_internal_node(int32_t nd, const TopoRec& t, const int3
     }
 }
 
-// Deep launch, one wave per block.  Blocks [0, ceil(nroots / R)) take R
-// consecutive roots at depth D each; seg[root * nlev + l] = the run of order[]
-// holding the root's subtree nodes at level l (depth D + l), and the runs of
-// consecutive roots at one level are adjacent, so the block's nodes at level l
-// are one run.
-//
-// Leaves first: each root's subtree is the id range [root, escape), taken 64
-// ids at a time, a lane per id.  The leaves among 64 consecutive ids (in
-// preorder) cover one run of sorted points with nothing between them, so the
-// run is staged through LDS T points at a time with coalesced loads (the
-// attributes are gathered by the permutation), the points' source records
-// written on the way; then each leaf's lane sums its points from LDS in order.
-// Every point is staged once.  Then the internal nodes, deepest level first,
-// with a barrier between levels (a node's children are one level deeper).
-//
-// The remaining blocks take the leaves among the first `shallow` positions of
-// the depth order (depths below D), a lane per leaf, writing their points'
-// source records themselves.
+// Leaf launch, one wave per 64 consecutive node ids (preorder).  The leaves
+// among them cover one run of sorted points with nothing between them (leaves
+// in preorder hold consecutive point ranges), so the run is staged through
+// LDS T points at a time with coalesced loads (the attributes are gathered by
+// the permutation), the points' source records written on the way; then each
+// leaf's lane sums its points from LDS in order.  Every point is staged once.
 template <typename TI, int FAM, int K, int P>
-__global__ __launch_bounds__(64, 2) void k_up_deep(
-    const int2* __restrict__ seg, int nlev, int32_t nroots, int32_t R, int32_t shallow,
-    const int32_t* __restrict__ order, const TopoRec* __restrict__ topo,
-    const int32_t* __restrict__ perm, const double* __restrict__ pts64,
-    const float* __restrict__ pts32, const TI* __restrict__ attrs,
-    const void* __restrict__ weights, int wdtype, int64_t m, double* __restrict__ w64,
-    double* __restrict__ r64, double* __restrict__ m64, void* __restrict__ nodes,
-    void* __restrict__ src, double an, int dir0) {
+__global__ __launch_bounds__(64, 2) void k_up_leaves(
+    int64_t m, const TopoRec* __restrict__ topo, const int32_t* __restrict__ perm,
+    const double* __restrict__ pts64, const float* __restrict__ pts32,
+    const TI* __restrict__ attrs, const void* __restrict__ weights, int wdtype,
+    double* __restrict__ w64, double* __restrict__ r64, double* __restrict__ m64,
+    void* __restrict__ nodes, void* __restrict__ src, double an, int dir0) {
 #pragma clang fp contract(off)
     constexpr int C = Rec<FAM, K, P>::C;
     constexpr int KC = K * C;
     constexpr int F = 3 + KC + K;  // staged per point: y, attributes, a weight per set
     constexpr int T = 256;         // points per staged tile
     constexpr int U = T / 64;
-    const int32_t nblk = (nroots + R - 1) / R;
-    const int32_t lane = (int32_t)threadIdx.x;
-    if ((int32_t)blockIdx.x >= nblk) {
-        const int32_t i = ((int32_t)blockIdx.x - nblk) * 64 + lane;
-        if (i < shallow) {
-            const int32_t nd = order[i];
-            const TopoRec t = topo[nd];
-            if (topo_leaf(t))
-                up_leaf_node<TI, FAM, K, P>(nd, t, perm, pts64, pts32, attrs, weights, wdtype, m,
-                                            w64, r64, m64, nodes, src, an, dir0);
-        }
-        return;
-    }
-    __shared__ int2 s_rng[kLevels + 1];
     __shared__ double s_pt[F * T];
-    const int32_t r0 = (int32_t)blockIdx.x * R, r1 = min(r0 + R, nroots);
-    if (lane < nlev)
-        s_rng[lane] = make_int2(seg[(int64_t)r0 * nlev + lane].x,
-                                seg[(int64_t)(r1 - 1) * nlev + lane].y);
-    __syncthreads();
-    for (int32_t q = s_rng[0].x; q < s_rng[0].y; ++q) {
-        const int32_t root = order[q];
-        const int32_t esc = topo[root].escape;
-        for (int32_t base = root; base < esc; base += 64) {
-            const int32_t nd = base + lane;
-            TopoRec t{};
-            bool leaf = false;
-            if (nd < esc) {
-                t = topo[nd];
-                leaf = topo_leaf(t);
-            }
-            const uint64_t lm = __ballot(leaf);
-            if (lm == 0) continue;
-            const int first = __ffsll((unsigned long long)lm) - 1;
-            const int last = 63 - __clzll((unsigned long long)lm);
-            const int32_t lo = __shfl(t.begin, first), hi = __shfl(t.end, last);
-            double tw[K], rep[K][3], acc[K][C], rep0[3] = {0.0, 0.0, 0.0};
-            for (int k = 0; k < K; ++k) {
-                tw[k] = 0.0;
-                for (int d = 0; d < 3; ++d) rep[k][d] = 0.0;
-                for (int c = 0; c < C; ++c) acc[k][c] = 0.0;
-            }
-            for (int32_t tl = lo; tl < hi; tl += T) {
-                const int32_t th = min(tl + T, hi);
-                PtIn<KC, P> pt[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int32_t s = min(tl + lane + 64 * u, th - 1);
-                    load_point<TI, KC, P>(pt[u], s, (int64_t)perm[s], pts64, pts32, attrs, weights,
-                                          wdtype);
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int32_t s = tl + lane + 64 * u;
-                    if (s < th) {
-                        Rec<FAM, K, P>::src(src, s, pt[u].y, pt[u].y32, pt[u].a, an, dir0);
-                        const int i = s - tl;
-                        for (int d = 0; d < 3; ++d) s_pt[d * T + i] = pt[u].y[d];
-                        for (int c = 0; c < KC; ++c) s_pt[(3 + c) * T + i] = pt[u].a[c];
-                        for (int k = 0; k < K; ++k) {
-                            double w;
-                            if (weights) {
-                                w = pt[u].w;
-                            } else {
-                                const double* a = pt[u].a + k * C;
-                                double ss = a[0] * a[0];
-                                for (int c = 1; c < C; ++c) ss = ss + a[c] * a[c];
-                                w = sqrt(ss);
-                            }
-                            s_pt[(3 + KC + k) * T + i] = w;
-                        }
-                    }
-                }
-                __syncthreads();
-                if (leaf) {
-                    const int32_t a1 = min(t.end, th);
-                    for (int32_t s = max(t.begin, tl); s < a1; ++s) {
-                        const int i = s - tl;
-                        const double y[3] = {s_pt[i], s_pt[T + i], s_pt[2 * T + i]};
-                        for (int k = 0; k < K; ++k) {
-                            const double w = s_pt[(3 + KC + k) * T + i];
-                            tw[k] += w;
-                            for (int c = 0; c < C; ++c) acc[k][c] += s_pt[(3 + k * C + c) * T + i];
-                            for (int d = 0; d < 3; ++d) rep[k][d] += y[d] * w;
-                        }
-                        for (int d = 0; d < 3; ++d) rep0[d] += y[d];
-                    }
-                }
-                __syncthreads();
-            }
-            if (leaf)
-                leaf_finish<FAM, K, P>(nd, t, tw, rep, acc, rep0, m, w64, r64, m64, nodes, an,
-                                       dir0);
-        }
+    const int32_t lane = (int32_t)threadIdx.x;
+    const int64_t nd = (int64_t)blockIdx.x * 64 + lane;
+    TopoRec t{};
+    bool leaf = false;
+    if (nd < m) {
+        t = topo[nd];
+        leaf = topo_leaf(t);
     }
-    for (int l = nlev - 1; l >= 0; --l) {
+    const uint64_t lm = __ballot(leaf);
+    if (lm == 0) return;
+    const int first = __ffsll((unsigned long long)lm) - 1;
+    const int last = 63 - __clzll((unsigned long long)lm);
+    const int32_t lo = __shfl(t.begin, first), hi = __shfl(t.end, last);
+    double tw[K], rep[K][3], acc[K][C], rep0[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < K; ++k) {
+        tw[k] = 0.0;
+        for (int d = 0; d < 3; ++d) rep[k][d] = 0.0;
+        for (int c = 0; c < C; ++c) acc[k][c] = 0.0;
+    }
+    for (int32_t tl = lo; tl < hi; tl += T) {
+        const int32_t th = min(tl + T, hi);
+        PtIn<KC, P> pt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t s = min(tl + lane + 64 * u, th - 1);
+            load_point<TI, KC, P>(pt[u], s, (int64_t)perm[s], pts64, pts32, attrs, weights, wdtype);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t s = tl + lane + 64 * u;
+            if (s < th) {
+                Rec<FAM, K, P>::src(src, s, pt[u].y, pt[u].y32, pt[u].a, an, dir0);
+                const int i = s - tl;
+                for (int d = 0; d < 3; ++d) s_pt[d * T + i] = pt[u].y[d];
+                for (int c = 0; c < KC; ++c) s_pt[(3 + c) * T + i] = pt[u].a[c];
+                for (int k = 0; k < K; ++k) {
+                    double w;
+                    if (weights) {
+                        w = pt[u].w;
+                    } else {
+                        const double* a = pt[u].a + k * C;
+                        double ss = a[0] * a[0];
+                        for (int c = 1; c < C; ++c) ss = ss + a[c] * a[c];
+                        w = sqrt(ss);
+                    }
+                    s_pt[(3 + KC + k) * T + i] = w;
+                }
+            }
+        }
         __syncthreads();
-        for (int32_t i = s_rng[l].x + lane; i < s_rng[l].y; i += 64) {
-            const int32_t nd = order[i];
-            const TopoRec t = topo[nd];
-            if (!topo_leaf(t)) {
-                int32_t ch[8];
-                load_kids(t, order, ch);
-                up_internal_node<FAM, K, P>(nd, t, ch, m, w64, r64, m64, nodes, an, dir0);
+        if (leaf) {
+            const int32_t a1 = min(t.end, th);
+            for (int32_t s = max(t.begin, tl); s < a1; ++s) {
+                const int i = s - tl;
+                const double y[3] = {s_pt[i], s_pt[T + i], s_pt[2 * T + i]};
+                for (int k = 0; k < K; ++k) {
+                    const double w = s_pt[(3 + KC + k) * T + i];
+                    tw[k] += w;
+                    for (int c = 0; c < C; ++c) acc[k][c] += s_pt[(3 + k * C + c) * T + i];
+                    for (int d = 0; d < 3; ++d) rep[k][d] += y[d] * w;
+                }
+                for (int d = 0; d < 3; ++d) rep0[d] += y[d];
             }
         }
+        __syncthreads();
     }
+    if (leaf)
+        leaf_finish<FAM, K, P>((int32_t)nd, t, tw, rep, acc, rep0, m, w64, r64, m64, nodes, an,
+                               dir0);
 }
 
 // Band launch, one wave per node at the band's top depth: the internal nodes
 // of the band's levels nlev-1..0 (nlev <= 3) under it, with a barrier between
-// levels; the leaves were done by the deep launch.  Level l holds at most 8^l
+// levels; the leaves were done by the leaf launch.  Level l holds at most 8^l
 // <= 64 of them, so each lane has at most one node per level, and the node ids
 // and child ids of the next level (which no launch here writes) are loaded
 // while the current level runs.
@@ -476,19 +371,13 @@ int run_upward(cagr_tree* t, const TI* attrs, const void* weights, int wdtype, d
                hipStream_t st) {
     const int64_t m = t->m;
     if (m == 0) return CAGR_OK;
-    const int deep_lev = t->depth - t->up_D + 1;
-    const int32_t R = t->up_group;
-    const int64_t nblk = (t->up_deep_roots + R - 1) / R + (t->up_shallow + 63) / 64;
-    bool ok = deep_lev >= 1 && deep_lev <= kLevels + 1 && t->up_D <= kLevels && R >= 1;
+    bool ok = true;
     for (const int4& b : t->up_bands) ok = ok && b.y >= 1 && b.y <= 3;  // <= 64 nodes a level
-    CAGR_REQUIRE(ok, CAGR_ERR_STATE, "upward: bad level split");
-    if (nblk > 0) {
-        k_up_deep<TI, FAM, K, P><<<(unsigned)nblk, 64, 0, st>>>(
-            t->up_seg, deep_lev, t->up_deep_roots, R, t->up_shallow, t->order, t->topo, t->perm,
-            t->pts64, t->pts32, attrs, weights, wdtype, m, t->w64, t->r64, t->m64, t->nodes,
-            t->src, an, dir0);
-        CAGR_LAUNCHED();
-    }
+    CAGR_REQUIRE(ok, CAGR_ERR_STATE, "upward: bad band");
+    k_up_leaves<TI, FAM, K, P><<<(unsigned)((m + 63) / 64), 64, 0, st>>>(
+        m, t->topo, t->perm, t->pts64, t->pts32, attrs, weights, wdtype, t->w64, t->r64, t->m64,
+        t->nodes, t->src, an, dir0);
+    CAGR_LAUNCHED();
     for (const int4& b : t->up_bands) {
         k_up_band<FAM, K, P><<<b.z, 64, 0, st>>>(t->up_seg + b.w, b.y, t->order, t->topo, m, t->w64,
                                                  t->r64, t->m64, t->nodes, an, dir0);
