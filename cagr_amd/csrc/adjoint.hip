@@ -585,7 +585,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_down(int64_t n, const int32_t*
 }  // namespace
 
 int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
-            int continuous, double an, void* out, hipStream_t st, bool planned) {
+            int continuous, double an, void* out, hipStream_t st, bool planned,
+            const void* nodes_in, const void* src_in) {
+    if (!nodes_in) nodes_in = t->nodes;
+    if (!src_in) src_in = t->src;
     AP p{};
     p.n = t->n;
     p.root = 0;
@@ -627,8 +630,8 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     const int64_t blocks = (n + 63) / 64;   // one wave per block
     p.tile_order = t->tile_order;
     if (precision == CAGR_PREC_FP32C) {
-        auto nodes = (const NodeV32*)t->nodes;
-        auto src = (const SrcV32*)t->src;
+        auto nodes = (const NodeV32*)nodes_in;
+        auto src = (const SrcV32*)src_in;
 #define CAGR_ADJ32(GMA, CT)                                                                      \
     do {                                                                                         \
         if (slots)                                                                               \
@@ -647,8 +650,8 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
         }
 #undef CAGR_ADJ32
     } else {
-        auto nodes = (const NodeV64*)t->nodes;
-        auto src = (const SrcV64*)t->src;
+        auto nodes = (const NodeV64*)nodes_in;
+        auto src = (const SrcV64*)src_in;
         if (ga) {
             if (continuous) k_adj64<1, true><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64);
             else k_adj64<1, false><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src, t->pts64);
@@ -683,6 +686,10 @@ void adjoint_plan_free(cagr_tree* t) {
     (void)hipFree(t->plan_near_off);
     (void)hipFree(t->plan_near_leaf);
     (void)hipFree(t->plan_near_mask);
+    (void)hipFree(t->plan_nodes);
+    (void)hipFree(t->plan_src);
+    t->plan_nodes = t->plan_src = nullptr;
+    t->plan_rec_an = __builtin_nan("");
     t->plan_w = nullptr;
     t->plan_wave_off = t->plan_node_off = t->plan_slot_pos = nullptr;
     t->plan_far_node = t->plan_near_off = t->plan_near_leaf = nullptr;

@@ -432,12 +432,33 @@ int cagr_apply_adjoint_planned(cagr_tree* t, int op, int precision, const void* 
     CAGR_REQUIRE(t->plan_slots >= 0, CAGR_ERR_STATE, "no adjoint plan: call cagr_adjoint_plan");
     CAGR_REQUIRE(g && widths && out, CAGR_ERR_ARG, "null g / widths / out");
     hipStream_t st = (hipStream_t)stream;
+    // the FP32 records of the plan's centres are kept after the first call
+    // (per convection factor); FP64 recomputes them
+    const double an_rec = oi.convects ? an : 0.0;
+    const bool keep = precision == CAGR_PREC_FP32C;
+    const bool kept = keep && t->plan_nodes && t->plan_rec_an == an_rec;
     if ((rc = prof_mark(t, 1, st))) return rc;
-    rc = upward(t, 0, precision, 1, 0, nullptr, CAGR_F64, t->plan_w, oi.convects ? an : 0.0, st);
-    if (rc) return rc;
+    if (!kept) {
+        rc = upward(t, 0, precision, 1, 0, nullptr, CAGR_F64, t->plan_w, an_rec, st);
+        if (rc) return rc;
+        if (keep) {
+            const size_t nb = (size_t)t->m * sizeof(NodeV32), sb = (size_t)t->n * sizeof(SrcV32);
+            if (!t->plan_nodes || !t->plan_src) {
+                (void)hipFree(t->plan_nodes);
+                (void)hipFree(t->plan_src);
+                t->plan_nodes = t->plan_src = nullptr;
+                CAGR_HIP(hipMalloc(&t->plan_nodes, nb));
+                CAGR_HIP(hipMalloc(&t->plan_src, sb));
+            }
+            CAGR_HIP(hipMemcpyAsync(t->plan_nodes, t->nodes, nb, hipMemcpyDeviceToDevice, st));
+            CAGR_HIP(hipMemcpyAsync(t->plan_src, t->src, sb, hipMemcpyDeviceToDevice, st));
+            t->plan_rec_an = an_rec;
+        }
+    }
     if ((rc = prof_mark(t, 1, st))) return rc;
     if ((rc = prof_mark(t, 0, st))) return rc;
-    rc = adjoint(t, op, precision, g, dtype, widths, continuous, an, out, st, true);
+    rc = adjoint(t, op, precision, g, dtype, widths, continuous, an, out, st, true,
+                 keep ? t->plan_nodes : nullptr, keep ? t->plan_src : nullptr);
     if (rc) return rc;
     return prof_mark(t, 0, st);
 }

@@ -135,6 +135,12 @@ struct cagr_tree {
     int32_t* plan_near_leaf = nullptr; // [near entries]
     uint64_t* plan_near_mask = nullptr;// [near entries]
     int64_t plan_slots = -1;           // -1: no plan
+    // FP32 node and source records of the plan's centres (an upward pass of
+    // plan_w with no attributes), kept after the first planned call with
+    // convection factor plan_rec_an (NaN: none kept)
+    void* plan_nodes = nullptr;
+    void* plan_src = nullptr;
+    double plan_rec_an = __builtin_nan("");
     int persistent = 0;              // 1: traversal through the item queue
     int waves_per_block = 4;         // launch shape of the persistent traversal kernels
     int grid_waves = 1;              // waves per block of the regular-grid traversal kernels
@@ -185,9 +191,11 @@ struct Queries {
 };
 // implemented in adjoint.hip (needs the node records of an upward pass with
 // the same fixed weights)
-// (planned: FP32 far steps go through the tree's adjoint plan instead of atomics)
+// (planned: FP32 far steps go through the tree's adjoint plan instead of atomics;
+// nodes / src: records to read instead of the tree's own, when not null)
 int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
-            int continuous, double an, void* out, hipStream_t st, bool planned = false);
+            int continuous, double an, void* out, hipStream_t st, bool planned = false,
+            const void* nodes = nullptr, const void* src = nullptr);
 // builds the adjoint plan for point weights (original order; centres of an
 // upward pass with them)
 int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st);

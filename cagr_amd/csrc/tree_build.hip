@@ -328,22 +328,13 @@ __global__ void k_depth_off(const uint64_t* __restrict__ keys, int64_t m, int32_
     off[d] = (int32_t)lo;
 }
 
-// First child's position in the depth order and child count of every
-// internal node, from its first child (positions of depth >= 1).  The
-// children of p are consecutive there: within a depth the order is ascending
-// id, and a node whose id lies between two children of p is inside p's
-// subtree, so at p's depth + 1 it is a child of p too.
+// child positions of the internal nodes (kids_body), one thread per position
+// of depth >= 1 (positions i0.. of the depth order)
 __global__ __launch_bounds__(kThreads) void k_kids(TopoRec* __restrict__ topo,
                                                    const int32_t* __restrict__ order, int32_t i0,
                                                    int64_t m) {
     const int64_t i = i0 + (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= m) return;
-    const int32_t p = topo[order[i]].parent;
-    if (topo[order[i - 1]].parent == p) return;  // not the first child
-    int64_t j = i + 1;
-    while (j < m && topo[order[j]].parent == p) ++j;
-    topo[p].kids = (int32_t)i;
-    topo[p].info = topo[p].info | ((int32_t)(j - i) << 12);
+    if (i < m) kids_body(i, topo, order, m);
 }
 
 // Segments of the upward launches: for root i (the id-sorted list of roots

@@ -208,6 +208,20 @@ CAGR_HD inline void node_body(int64_t b, const uint64_t* keys, int64_t n, const 
     }
 }
 
+// First child's position in the depth order (node ids sorted by depth,
+// ascending id within a depth) and child count of every internal node, from
+// the position i >= 1 of each node of depth >= 1.  The children of p are
+// consecutive there: a node whose id lies between two children of p is inside
+// p's subtree, so at p's depth + 1 it is a child of p too.
+CAGR_HD inline void kids_body(int64_t i, TopoRec* topo, const int32_t* order, int64_t m) {
+    const int32_t p = topo[order[i]].parent;
+    if (topo[order[i - 1]].parent == p) return;  // not the first child
+    int64_t j = i + 1;
+    while (j < m && topo[order[j]].parent == p) ++j;
+    topo[p].kids = (int32_t)i;
+    topo[p].info = topo[p].info | ((int32_t)(j - i) << 12);
+}
+
 // leaves list their points in ascending original index: insertion-sort the
 // leaf's slice of the permutation (already sorted within equal keys)
 CAGR_HD inline void leaf_sort_body(const TopoRec& t, int32_t* perm) {

@@ -1,8 +1,9 @@
 // CPU replay of the HIP tree build (test only).  Compiles the shared
 // per-element logic of cagr_amd/csrc/tree_logic.h with g++ and runs the same
 // sequence the kernels run -- keys, stable sort, chains, scan, node records,
-// leaf sorts -- serially, then writes the reference serialised layout so the
-// test can compare it with the oracle tree.
+// leaf sorts, the depth order and the child positions -- serially, then writes
+// the reference serialised layout so the test can compare it with the oracle
+// tree.
 #define CAGR_HD
 #include "../../cagr_amd/csrc/tree_logic.h"
 #include "../../cagr_amd/csrc/tree_export.h"
@@ -84,4 +85,21 @@ extern "C" int64_t hl_build_forest(const double* pts, const int64_t* coff, int64
     if (roots_out)
         for (int64_t c = 0; c <= nclouds; ++c) roots_out[c] = off[coff[c]];
     return m;
+}
+
+// Depth order (node ids stably sorted by depth) and the child positions of
+// every internal node (kids_body over the positions of depth >= 1) of the
+// records topo[m] of hl_build / hl_build_forest; writes order[m],
+// kids[m] (-1: none) and nkids[m].
+extern "C" void hl_kids(TopoRec* topo, int64_t m, int32_t* order, int32_t* kids, int32_t* nkids) {
+    std::iota(order, order + m, 0);
+    std::stable_sort(order, order + m,
+                     [&](int32_t a, int32_t b) { return topo_depth(topo[a]) < topo_depth(topo[b]); });
+    int64_t i0 = 0;
+    while (i0 < m && topo_depth(topo[order[i0]]) == 0) ++i0;
+    for (int64_t i = i0; i < m; ++i) kids_body(i, topo, order, m);
+    for (int64_t i = 0; i < m; ++i) {
+        kids[i] = topo[i].kids;
+        nkids[i] = topo_nkids(topo[i]);
+    }
 }

@@ -297,3 +297,24 @@ def test_schedule_an1_matches_oracle_replay():
     nrm, res = cagr_schedule(wt, 0.01, 0.04, precision="fp64")
     onrm, ores = solve_ref.cagr_schedule(p, 0.01, 0.04)
     assert normwise(nrm.cpu().numpy(), onrm) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_orient_end_to_end_reproduces_bunny(tmp_path, precision):
+    """cagr_amd.pcio.orient -- read .xyz, normalize, float32 points on the GPU
+    (main_CAGR.py:47), the 40-iteration schedule and 8 G steps, write .xyz --
+    reproduces the reference-held bunny normals (written with An = 0)."""
+    from cagr_amd import pcio
+    say("orient bunny", precision)
+    d = np.load(os.path.join(GOLD, "bunny.npz"))
+    src = str(tmp_path / "bunny.xyz")
+    np.savetxt(src, d["points"])
+    out_dir = str(tmp_path / "out")
+    nrm, pn = pcio.orient(src, "l0", an=0.0, precision=precision, out_dir=out_dir)
+    assert np.array_equal(pn, pcio.normalize(d["points"])[0])
+    ref = d["normals"]
+    cos = (nrm * ref).sum(1) / np.linalg.norm(nrm, axis=1) / np.linalg.norm(ref, axis=1)
+    mag = np.linalg.norm(nrm, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(cos) > 0.9999 and (cos > 0).mean() == 1.0 and abs(np.median(mag) - 1) < 0.02
+    pts, nrm2 = pcio.load_points(os.path.join(out_dir, "bunny.xyz"))
+    assert np.array_equal(pts, d["points"]) and np.array_equal(nrm2, nrm.astype(np.float64))

@@ -113,6 +113,13 @@ def test_planned_adjoint_matches_atomic_and_is_deterministic(op):
     wt.apply("A2", mu, widths, precision="fp32c")
     c = wt.apply_adjoint_planned(op, g, widths, precision="fp32c")
     assert torch.equal(a, c)
+    # the plan keeps the FP32 records of its centres per convection factor:
+    # another An gives that An's adjoint, and switching back gives a again
+    if op in ("A2", "A3"):
+        ref0 = wt.apply_adjoint(op, g, widths, w1, an=0.5, precision="fp32c")
+        d = wt.apply_adjoint_planned(op, g, widths, an=0.5, precision="fp32c")
+        assert rel(d, ref0) <= 1e-6 and rel(d, a) > 1e-3
+        assert torch.equal(wt.apply_adjoint_planned(op, g, widths, precision="fp32c"), a)
     # FP64 with the plan's weights is the plain FP64 adjoint
     r64 = wt.apply_adjoint(op, g, widths, w1, precision="fp64")
     p64 = wt.apply_adjoint_planned(op, g, widths, precision="fp64")

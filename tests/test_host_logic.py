@@ -138,3 +138,83 @@ def test_forest_replay_equals_per_cloud_trees(hl):
             assert np.array_equal(topo[r0:r1].view(np.float64).reshape(-1, 4)[:, 0],
                                   t1.view(np.float64).reshape(-1, 4)[:, 0])  # thr2
             assert np.array_equal(perm[p0:p0 + n_c] - p0, perm1)
+
+
+def hl_topo(L, p, md, S, theta=0.5):
+    p = np.ascontiguousarray(p, np.float64)
+    P = ct.c_void_p
+    m = np.zeros(1, np.int64)
+    L.hl_build(p.ctypes.data_as(P), p.shape[0], md, S, theta, m.ctypes.data_as(P), None, None)
+    topo = np.zeros((max(int(m[0]), 1), 32), np.uint8)
+    perm = np.zeros(max(p.shape[0], 1), np.int32)
+    L.hl_build(p.ctypes.data_as(P), p.shape[0], md, S, theta, m.ctypes.data_as(P),
+               topo.ctypes.data_as(P), perm.ctypes.data_as(P))
+    return topo, int(m[0])
+
+
+def hl_kids(L, topo, m):
+    P = ct.c_void_p
+    L.hl_kids.argtypes = [P, ct.c_int64, P, P, P]
+    L.hl_kids.restype = None
+    order, kids, nk = (np.zeros(max(m, 1), np.int32) for _ in range(3))
+    L.hl_kids(topo.ctypes.data_as(P), m, order.ctypes.data_as(P), kids.ctypes.data_as(P),
+              nk.ctypes.data_as(P))
+    return order[:m], kids[:m], nk[:m]
+
+
+@pytest.mark.parametrize("name", list(_cases()))
+@pytest.mark.parametrize("S", [1, 32])
+def test_child_positions_match_oracle_children(hl, name, S):
+    """The upward pass reads a node's children at positions kids .. kids +
+    nkids of the depth order: they must be exactly the oracle tree's children
+    in octant order, and leaves have none."""
+    p = _cases()[name]
+    a = oracle.build_tree(p, 15, S)
+    topo, m = hl_topo(hl, p, 15, S)
+    assert m == a.num_nodes
+    if m == 0:
+        return
+    order, kids, nk = hl_kids(hl, topo, m)
+    depth = np.round(-np.log2(a.half_w)).astype(np.int64)
+    assert np.all(np.diff(depth[order]) >= 0)
+    for d in np.unique(depth):
+        ids = order[depth[order] == d]
+        assert np.all(np.diff(ids) > 0)     # ascending id within a depth
+    for v in range(m):
+        c = a.children[v][a.children[v] >= 0]
+        if a.is_leaf[v]:
+            assert kids[v] == -1 and nk[v] == 0
+        else:
+            assert 1 <= nk[v] <= 8
+            assert np.array_equal(order[kids[v]:kids[v] + nk[v]], c), v
+
+
+def test_child_positions_in_a_forest(hl):
+    """In a forest (several clouds, one root each) every non-root node is
+    listed once, under its own parent, children in ascending id."""
+    P = ct.c_void_p
+    hl.hl_build_forest.argtypes = [P, P, ct.c_int64, ct.c_int, ct.c_int, ct.c_double, P, P, P]
+    hl.hl_build_forest.restype = ct.c_int64
+    rng = np.random.default_rng(9)
+    clouds = [synth.normalize(synth.sphere(700, 3)), rng.uniform(-1, 1, (4, 3)),
+              synth.normalize(synth.torus(900, 5)), rng.uniform(-1, 1, (1, 3))]
+    coff = np.cumsum([0] + [c.shape[0] for c in clouds]).astype(np.int64)
+    pts = np.ascontiguousarray(np.concatenate(clouds), np.float64)
+    for S in (1, 8):
+        m = hl.hl_build_forest(pts.ctypes.data_as(P), coff.ctypes.data_as(P), len(clouds), 15, S,
+                               0.5, None, None, None)
+        topo = np.zeros((m, 32), np.uint8)
+        perm = np.zeros(pts.shape[0], np.int32)
+        roots = np.zeros(len(clouds) + 1, np.int32)
+        hl.hl_build_forest(pts.ctypes.data_as(P), coff.ctypes.data_as(P), len(clouds), 15, S, 0.5,
+                           topo.ctypes.data_as(P), perm.ctypes.data_as(P), roots.ctypes.data_as(P))
+        parent = topo.view(np.int32).reshape(m, 8)[:, 6].copy()
+        order, kids, nk = hl_kids(hl, topo, m)
+        seen = np.zeros(m, np.int64)
+        for v in range(m):
+            if nk[v]:
+                c = order[kids[v]:kids[v] + nk[v]]
+                assert np.all(parent[c] == v) and np.all(np.diff(c) > 0)
+                seen[c] += 1
+        assert np.array_equal(seen, (parent >= 0).astype(np.int64))
+        assert (parent < 0).sum() == len(clouds)
