@@ -105,6 +105,71 @@ __device__ inline const T& rec32(const T* base, int32_t i) {
     return *(const T*)((const char*)base + (uint64_t)((uint32_t)i * (uint32_t)sizeof(T)));
 }
 
+// ---- spreading a near leaf over the idle lanes ----
+// At a leaf where only na lanes are near, lane L works for slot q = L >> lr
+// (the near lane of rank q: the q-th set bit of the near mask) as group
+// g = L & (2^lr - 1) of that slot, taking the leaf's points g, g + 2^lr, ...;
+// the 2^lr partial sums of a slot are then added in a fixed butterfly and the
+// near lane of rank q takes the slot's total.  All 64 lanes must be active in
+// these helpers.
+struct Spread {
+    int lr;      // log2 of the lanes per slot
+    int g;       // this lane's group within its slot
+    bool part;   // this lane works for a slot (L < na << lr)
+    int owner;   // the near lane whose query this lane's slot evaluates
+    int rank;    // this lane's rank among the near lanes (meaningful on them)
+};
+
+// lanes per slot (log2): as many as fit next to na near lanes, at most 16,
+// and not more than the leaf's points; 0 = do not spread (small leaves)
+__device__ inline int spread_log2(int na, int npts) {
+    int lr = 0;
+    if (npts < 4) return 0;
+    while (lr < 4 && (na << (lr + 1)) <= 64 && (2 << lr) <= npts) ++lr;
+    return lr;
+}
+
+template <typename T>
+__device__ inline T lane_pull(T v, int src_lane) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_ds_bpermute(src_lane << 2, __builtin_bit_cast(int, v)));
+    } else {
+        static_assert(sizeof(T) == 8, "32- or 64-bit values only");
+        const int64_t b = __builtin_bit_cast(int64_t, v);
+        const int lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b & 0xffffffff));
+        const int hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, (int)(b >> 32));
+        return __builtin_bit_cast(T, ((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+    }
+}
+
+__device__ inline Spread spread_setup(uint64_t nm, bool nearl, int lr) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int na = __popcll(nm);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(nm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
+    // near lanes send their id to lane `rank`, the others to lanes na.. (a
+    // permutation of the 64 lanes), so lane q < na receives the id of slot q
+    const int dest = nearl ? rank : na + (lane - rank);
+    const int own = __builtin_amdgcn_ds_permute(dest << 2, lane);
+    Spread s;
+    s.lr = lr;
+    s.g = lane & ((1 << lr) - 1);
+    const int q = lane >> lr;
+    s.part = q < na;
+    s.owner = __builtin_amdgcn_ds_bpermute((s.part ? q : 0) << 2, own);
+    s.rank = rank;
+    return s;
+}
+
+// sum over the 2^lr lanes of a slot (every lane of the slot gets the same
+// total: each butterfly step adds the same two values in either lane)
+template <typename T>
+__device__ inline T spread_sum(T v, int lr) {
+    for (int s = 1; s < (1 << lr); s <<= 1) v += __shfl_xor(v, s, 64);
+    return v;
+}
+
 // lane `lane`'s 64-bit value to every lane (a scalar)
 __device__ inline uint64_t lane_bcast_u64(uint64_t v, int lane) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);

@@ -290,7 +290,30 @@ __device__ __forceinline__ void trav_v(
                 } else if (__ballot(nearl) != 0) {
                     const bool sure = leaf_surely_outside(xf, p.lbound[n], wf);
                     const bool fast = __ballot(nearl && !sure) == 0;
-                    if (nearl) {
+                    const uint64_t nm = __ballot(nearl);
+                    const int lr = fast ? spread_log2(__popcll(nm), t.end - t.begin) : 0;
+                    if (lr > 0) {   // few near lanes: the idle lanes share the points
+                        const Spread sp = spread_setup(nm, nearl, lr);
+                        float qx[3], ql[3] = {0.f, 0.f, 0.f}, part[3] = {0.f, 0.f, 0.f};
+                        for (int k = 0; k < 3; ++k) qx[k] = lane_pull(xf[k], sp.owner);
+                        if constexpr (QM != QM_SELF)
+                            for (int k = 0; k < 3; ++k) ql[k] = lane_pull(xl[k], sp.owner);
+                        if (sp.part) {
+                            for (int32_t j = t.begin + sp.g; j < t.end; j += 1 << lr) {
+                                const SrcV32 sr = src[j];
+                                float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
+                                if constexpr (QM != QM_SELF)
+                                    for (int k = 0; k < 3; ++k) d[k] += ql[k];
+                                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                                acc.add(part, d, r2, false, sr.mu, sr.e, sr.q);
+                            }
+                        }
+                        for (int k = 0; k < acc.NO; ++k) {
+                            const float v = lane_pull(spread_sum(part[k], lr), sp.rank << lr);
+                            if (nearl) acc.lp[k] += v;
+                        }
+                        if (nearl) acc.flush_leaf();
+                    } else if (nearl) {
                         if (fast) {   // no pair of this leaf is within any lane's width
                             for (int32_t j = t.begin; j < t.end; ++j) {
                                 const SrcV32 sr = src[j];
@@ -345,11 +368,40 @@ __device__ __forceinline__ void trav_v(
             }
             const bool nearl = act && !far;
             if (topo_leaf(t)) {
-                if (nearl) {
-                    resume = t.escape;
-                    if constexpr (CNT) {
-                        nnear += t.end - t.begin;
-                    } else {
+                if (nearl) resume = t.escape;
+                if constexpr (CNT) {
+                    if (nearl) nnear += t.end - t.begin;
+                } else if (const uint64_t nm = __ballot(nearl); nm != 0) {
+                    const int lr = spread_log2(__popcll(nm), t.end - t.begin);
+                    if (lr > 0) {   // few near lanes: the idle lanes share the points
+                        const Spread sp = spread_setup(nm, nearl, lr);
+                        double qx[3];
+                        for (int k = 0; k < 3; ++k) qx[k] = lane_pull(x[k], sp.owner);
+                        const double qw = lane_pull(w, sp.owner);
+                        VAcc64<VOP, CONT> part;
+                        part.init();
+                        if (sp.part) {
+                            for (int32_t j = t.begin + sp.g; j < t.end; j += 1 << lr) {
+                                const SrcV64 sr = src[j];
+                                const double dd[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1],
+                                                      qx[2] - sr.y[2]};
+                                part.add(dd, qw, sr.mu, p.an);
+                            }
+                        }
+                        const int from = sp.rank << lr;
+                        for (int k = 0; k < acc.NO; ++k) {
+                            const double v = lane_pull(spread_sum(part.s[k], lr), from);
+                            if (nearl) acc.s[k] += v;
+                        }
+                        if constexpr (CONT) {
+                            const double v = lane_pull(spread_sum(part.P, lr), from);
+                            if (nearl) acc.P += v;
+                            for (int k = 0; k < 3; ++k) {
+                                const double u = lane_pull(spread_sum(part.Q[k], lr), from);
+                                if (nearl) acc.Q[k] += u;
+                            }
+                        }
+                    } else if (nearl) {
                         for (int32_t j = t.begin; j < t.end; ++j) {
                             const SrcV64 sr = src[j];
                             const double dd[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
@@ -439,8 +491,8 @@ __device__ __forceinline__ void trav_s(
             far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
         double s[3] = {0.0, 0.0, 0.0};   // FP64 totals of the FP32 partials
         float lp[3] = {0.f, 0.f, 0.f}, fp[3] = {0.f, 0.f, 0.f};
-        // one near-pair interaction of all K sets (weights mk), into lp
-        auto near_pair = [&](const SrcS32<K>& sr, const float* d, float r2, bool in,
+        // one near-pair interaction of all K sets (weights mk), into dst
+        auto near_pair = [&](float* dst, const SrcS32<K>& sr, const float* d, float r2, bool in,
                              const float* mk) {
             const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
             const float ir3 = ir * ir * ir;
@@ -451,14 +503,14 @@ __device__ __forceinline__ void trav_s(
 #pragma unroll
             for (int k = 1; k < K; ++k) gsum += g[k];
             const float gi3 = gsum * ir3;
-            for (int cc = 0; cc < 3; ++cc) lp[cc] = fmaf(gi3, d[cc], lp[cc]);
+            for (int cc = 0; cc < 3; ++cc) dst[cc] = fmaf(gi3, d[cc], dst[cc]);
             const float hir = hanf * ir;
             if constexpr (K == 3) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) lp[k] = fmaf(g[k], hir, lp[k]);
+                for (int k = 0; k < 3; ++k) dst[k] = fmaf(g[k], hir, dst[k]);
             } else {
                 const float v = g[0] * hir;
-                for (int cc = 0; cc < 3; ++cc) lp[cc] += (cc == p.kdir) ? v : 0.f;
+                for (int cc = 0; cc < 3; ++cc) dst[cc] += (cc == p.kdir) ? v : 0.f;
             }
         };
         for (int32_t n = root; n < stop;) {
@@ -511,7 +563,38 @@ __device__ __forceinline__ void trav_s(
                 } else if (__ballot(any_near) != 0) {
                     const bool sure = leaf_surely_outside(xf, p.lbound[n], wf);
                     const bool fast = __ballot(any_near && !sure) == 0;
-                    if (any_near) {
+                    const uint64_t nm = __ballot(any_near);
+                    const int lr = fast ? spread_log2(__popcll(nm), t.end - t.begin) : 0;
+                    if (lr > 0) {   // few near lanes: the idle lanes share the points
+                        const Spread sp = spread_setup(nm, any_near, lr);
+                        float mk[K], qx[3], ql[3] = {0.f, 0.f, 0.f}, part[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            mk[k] = lane_pull(nearl[k] ? eq[k] : 0.f, sp.owner);
+                        for (int k = 0; k < 3; ++k) qx[k] = lane_pull(xf[k], sp.owner);
+                        if constexpr (QM != QM_SELF)
+                            for (int k = 0; k < 3; ++k) ql[k] = lane_pull(xl[k], sp.owner);
+                        if (sp.part) {
+                            for (int32_t j = t.begin + sp.g; j < t.end; j += 1 << lr) {
+                                const SrcS32<K> sr = src[j];
+                                float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
+                                if constexpr (QM != QM_SELF)
+                                    for (int k = 0; k < 3; ++k) d[k] += ql[k];
+                                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                                near_pair(part, sr, d, r2, false, mk);
+                            }
+                        }
+                        for (int cc = 0; cc < 3; ++cc) {
+                            const float v = lane_pull(spread_sum(part[cc], lr), sp.rank << lr);
+                            if (any_near) lp[cc] += v;
+                        }
+                        if (any_near)
+                            for (int cc = 0; cc < 3; ++cc) {
+                                s[cc] += (double)lp[cc] + (double)fp[cc];
+                                lp[cc] = 0.f;
+                                fp[cc] = 0.f;
+                            }
+                    } else if (any_near) {
                         float mk[K];
 #pragma unroll
                         for (int k = 0; k < K; ++k) mk[k] = nearl[k] ? eq[k] : 0.f;
@@ -522,7 +605,7 @@ __device__ __forceinline__ void trav_s(
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
                                 const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                                near_pair(sr, d, r2, false, mk);
+                                near_pair(lp, sr, d, r2, false, mk);
                             }
                         } else {
                             for (int32_t j = t.begin; j < t.end; ++j) {
@@ -534,7 +617,7 @@ __device__ __forceinline__ void trav_s(
                                 const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
                                                             x[1] - (double)sr.y[1],
                                                             x[2] - (double)sr.y[2], w);
-                                near_pair(sr, d, r2, in, mk);
+                                near_pair(lp, sr, d, r2, in, mk);
                             }
                         }
                         for (int cc = 0; cc < 3; ++cc) {
@@ -559,18 +642,20 @@ __device__ __forceinline__ void trav_s(
         const NodeS64* __restrict__ nodes = (const NodeS64*)nodes_;
         const SrcS64<K>* __restrict__ src = (const SrcS64<K>*)src_;
         double acc[3] = {0.0, 0.0, 0.0};
-        // one A2T_dir interaction (AT when an = 0) of scalar sv at d
-        auto add = [&](const double* d, double sv, int dir) {
+        // one A2T_dir interaction (AT when an = 0) of scalar sv at d, width wq,
+        // into dst
+        auto add_to = [&](double* dst, const double* d, double sv, int dir, double wq) {
             const double r2 = dist2_rn(d[0], d[1], d[2]);
             const double r = sqrt(r2);
-            if (r >= w) {
+            if (r >= wq) {
                 const double ir = 1.0 / r;
                 const double g = exp(han * sel3(d, dir)) * sv;
                 const double gi3 = g * (ir * ir * ir);
                 const double hg = han * g * ir;
-                for (int cc = 0; cc < 3; ++cc) acc[cc] += gi3 * d[cc] + (cc == dir ? hg : 0.0);
+                for (int cc = 0; cc < 3; ++cc) dst[cc] += gi3 * d[cc] + (cc == dir ? hg : 0.0);
             }
         };
+        auto add = [&](const double* d, double sv, int dir) { add_to(acc, d, sv, dir, w); };
         for (int32_t n = root; n < stop;) {
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = rec32(topo, n);
@@ -591,12 +676,34 @@ __device__ __forceinline__ void trav_s(
                 any_near |= nearl[k];
             }
             if (topo_leaf(t)) {
-                if (any_near) {
-                    for (int k = 0; k < K; ++k)
-                        if (nearl[k]) resume[k] = t.escape;
-                    if constexpr (CNT) {
-                        for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
-                    } else {
+                for (int k = 0; k < K; ++k)
+                    if (nearl[k]) resume[k] = t.escape;
+                if constexpr (CNT) {
+                    for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
+                } else if (const uint64_t nm = __ballot(any_near); nm != 0) {
+                    const int lr = spread_log2(__popcll(nm), t.end - t.begin);
+                    if (lr > 0) {   // few near lanes: the idle lanes share the points
+                        const Spread sp = spread_setup(nm, any_near, lr);
+                        int bits = 0;
+                        for (int k = 0; k < K; ++k) bits |= nearl[k] ? 1 << k : 0;
+                        const int qbits = lane_pull(bits, sp.owner);
+                        double qx[3], part[3] = {0.0, 0.0, 0.0};
+                        for (int k = 0; k < 3; ++k) qx[k] = lane_pull(x[k], sp.owner);
+                        const double qw = lane_pull(w, sp.owner);
+                        if (sp.part) {
+                            for (int32_t j = t.begin + sp.g; j < t.end; j += 1 << lr) {
+                                const SrcS64<K> sr = src[j];
+                                const double d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1],
+                                                     qx[2] - sr.y[2]};
+                                for (int k = 0; k < K; ++k)
+                                    if (qbits & (1 << k)) add_to(part, d, sr.s[k], p.kdir + k, qw);
+                            }
+                        }
+                        for (int cc = 0; cc < 3; ++cc) {
+                            const double v = lane_pull(spread_sum(part[cc], lr), sp.rank << lr);
+                            if (any_near) acc[cc] += v;
+                        }
+                    } else if (any_near) {
                         for (int32_t j = t.begin; j < t.end; ++j) {
                             const SrcS64<K> sr = src[j];
                             const double d[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
