@@ -124,37 +124,58 @@ __device__ inline void adj_term32(float* z, const float* d, float r2, bool use, 
     }
 }
 
-// The transposed near block of one leaf for the wave's near lanes qmask:
-// lanes hold the leaf's sources; for leaves of <= 32 points both wave halves
-// hold the same 32 sources and take alternate queries of the mask, so every
-// lane works, and the halves' sums meet through one lane shift.  One FP64
-// atomic per source and component into N.  All 64 lanes must be active.
+// Group layout of a transposed near block (lanes hold a leaf's sources):
+// groups of 2^lg lanes, the smallest power of two holding the leaf's points
+// (64 lanes and chunks of 64 points for leaves of more than 32), and the
+// 64 >> lg groups take consecutive near queries of the wave, so a leaf of 14
+// points serves 4 queries per step.  inv: lane r < nq holds the lane of the
+// r-th near query (a permutation sent through ds_permute).  All 64 lanes must
+// be active.
+struct NearGroups {
+    int lg, grp, nq, inv;
+};
+
+__device__ inline NearGroups near_groups(uint64_t qmask, int32_t npts, int lane) {
+    NearGroups g;
+    g.lg = 0;
+    while ((1 << g.lg) < npts && g.lg < 6) ++g.lg;
+    g.grp = lane >> g.lg;
+    g.nq = __popcll(qmask);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(qmask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)qmask, 0u));
+    const bool isq = (qmask >> lane) & 1;
+    g.inv = __builtin_amdgcn_ds_permute((isq ? rank : g.nq + (lane - rank)) << 2, lane);
+    return g;
+}
+
+// lane of the near query this lane's group takes in the step starting at
+// rank b0 (ok = false past the last query)
+__device__ inline int near_query(const NearGroups& g, int b0, bool& ok) {
+    const int r = b0 + g.grp;
+    ok = r < g.nq;
+    return __builtin_amdgcn_ds_bpermute((ok ? r : 0) << 2, g.inv);
+}
+
+// The transposed near block of one leaf for the wave's near lanes qmask
+// (group layout above); the groups' sums meet in group 0 through a butterfly.
+// One FP64 atomic per source and component into N.  All 64 lanes must be
+// active.
 template <bool GA, bool CONT>
 __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qmask, int lane,
                                      const SrcV32* __restrict__ src, const double* x,
                                      const float* xf, const float* h, const float* gin, float w2,
                                      double w, float winv, float ew, float hanf) {
-    const bool split = t.end - t.begin <= 32;
-    const int half = lane >> 5;
-    for (int32_t base = t.begin; base < t.end; base += split ? 32 : 64) {
-        const int32_t j = base + (split ? (lane & 31) : lane);
+    const NearGroups ng = near_groups(qmask, t.end - t.begin, lane);
+    const int S = 1 << ng.lg, G = 64 >> ng.lg;
+    for (int32_t base = t.begin; base < t.end; base += S) {
+        const int32_t j = base + (lane & (S - 1));
         const bool has = j < t.end;
         const SrcV32 sr = src[has ? j : t.begin];   // lanes past the leaf read a real record
         float zz[3] = {0.f, 0.f, 0.f};
-        uint64_t mm = qmask;
-        while (mm != 0) {
-            const int qi = __builtin_ctzll(mm);
-            mm &= mm - 1;
-            int qj = qi;       // the upper half's query (split mode)
-            bool qj_ok = !split;
-            if (split && mm != 0) {
-                qj = __builtin_ctzll(mm);
-                mm &= mm - 1;
-                qj_ok = true;
-            }
-            // the lane's query value, fetched through the LDS
-            // crossbar (ds_bpermute; every lane is active here)
-            const int qsrc = (split && half) ? qj : qi;
+        for (int b0 = 0; b0 < ng.nq; b0 += G) {
+            bool qok;
+            const int qsrc = near_query(ng, b0, qok);
+            // the group's query values, fetched through the LDS crossbar
             auto pick = [&](auto v) { return __shfl(v, qsrc, 64); };
             const float qx[3] = {pick(xf[0]), pick(xf[1]), pick(xf[2])};
             const float qh[3] = {pick(h[0]), pick(h[1]), pick(h[2])};
@@ -167,7 +188,7 @@ __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qma
                 qwinv = pick(winv);
                 qew = pick(ew);
             }
-            const bool use = has && (!split || !half || qj_ok);
+            const bool use = has && qok;
             const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
             const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
             const bool amb = use && fabsf(r2 - qw2) <= 1e-5f * qw2;
@@ -182,10 +203,9 @@ __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qma
             }
             adj_term32<GA, CONT>(zz, d, r2, use, in, qh, sr.e, hanf, qg, qwinv, qew);
         }
-        if (split) {   // upper half adds its sums into the lower half
-            for (int k = 0; k < 3; ++k) zz[k] += __shfl_down(zz[k], 32, 64);
-        }
-        if (has && (!split || !half)) {
+        for (int sh = S; sh < 64; sh <<= 1)
+            for (int k = 0; k < 3; ++k) zz[k] += __shfl_xor(zz[k], sh, 64);
+        if (has && ng.grp == 0) {
             atomicAdd(p.adj_pt + j, (double)zz[0]);
             atomicAdd(p.adj_pt + p.n + j, (double)zz[1]);
             atomicAdd(p.adj_pt + 2 * p.n + j, (double)zz[2]);
@@ -527,26 +547,30 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
         if (topo_leaf(t)) {
             if (nearl) resume = t.escape;
             const uint64_t qmask = __ballot(nearl);
-            if (qmask != 0) {
-                for (int32_t base = t.begin; base < t.end; base += 64) {
-                    const int32_t j = base + lane;
+            if (qmask != 0) {   // transposed, in the group layout of near_groups
+                const NearGroups ng = near_groups(qmask, t.end - t.begin, lane);
+                const int S = 1 << ng.lg, G = 64 >> ng.lg;
+                for (int32_t base = t.begin; base < t.end; base += S) {
+                    const int32_t j = base + (lane & (S - 1));
                     const bool has = j < t.end;
-                    SrcV64 sr;
-                    if (has) sr = src[j];
+                    const SrcV64 sr = src[has ? j : t.begin];
                     double zz[3] = {0.0, 0.0, 0.0};
-                    for (uint64_t mm = qmask; mm != 0; mm &= mm - 1) {
-                        const int qi = __builtin_ctzll(mm);
-                        const double qx[3] = {lane_bcast(x[0], qi), lane_bcast(x[1], qi),
-                                              lane_bcast(x[2], qi)};
-                        const double qg[3] = {lane_bcast(gk[0], qi), lane_bcast(gk[1], qi),
-                                              lane_bcast(gk[2], qi)};
-                        const double qw = lane_bcast(w, qi);
-                        if (has) {
+                    for (int b0 = 0; b0 < ng.nq; b0 += G) {
+                        bool qok;
+                        const int qsrc = near_query(ng, b0, qok);
+                        const double qx[3] = {lane_pull(x[0], qsrc), lane_pull(x[1], qsrc),
+                                              lane_pull(x[2], qsrc)};
+                        const double qg[3] = {lane_pull(gk[0], qsrc), lane_pull(gk[1], qsrc),
+                                              lane_pull(gk[2], qsrc)};
+                        const double qw = lane_pull(w, qsrc);
+                        if (has && qok) {
                             const double dd[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
                             adj_term64<GA, CONT>(zz, dd, qw, qg, p.an, p.sign);
                         }
                     }
-                    if (has) {
+                    for (int sh = S; sh < 64; sh <<= 1)
+                        for (int k = 0; k < 3; ++k) zz[k] += __shfl_xor(zz[k], sh, 64);
+                    if (has && ng.grp == 0) {
                         atomicAdd(p.adj_pt + j, zz[0]);
                         atomicAdd(p.adj_pt + p.n + j, zz[1]);
                         atomicAdd(p.adj_pt + 2 * p.n + j, zz[2]);
