@@ -192,7 +192,7 @@ def main():
     flop_a = inter_a * 3 * F_OP["A2k"]          # three directions per interaction
     flop_t = inter_t * F_OP["A2Tk"]             # interactions already summed over the 3 sets
     dom = "A2" if t_a2 >= t_a2t else "A2T_sum"
-    dom_kernel = "k_trav_v<1, 2, 0, false, false>" if dom == "A2" else "k_trav_s<1, 3, 0, false>"
+    dom_kernel = "k_trav_v32<2, 0, false>" if dom == "A2" else "k_trav_s32<3, 0>"
     dom_flop, dom_ms = (flop_a, t_a2) if dom == "A2" else (flop_t, t_a2t)
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
     log(f"interactions per launch: A2 {inter_a:.3e}, A2T_sum {inter_t:.3e}; dominant {dom}: "

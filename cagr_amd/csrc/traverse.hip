@@ -754,6 +754,32 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
     trav_s<P, K, QM, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root, p.stop);
 }
 
+// The FP32c walks over the tree's own points, held to 64 VGPRs so that 8
+// waves fit per SIMD: a wave walks for about a third of the launch, and more
+// resident waves shorten the launch's tail (A2 -5 % at 1M points).  The FP64
+// walks, the continuous kernel and the other query modes would spill.
+template <int VOP, int QM, bool CONT>
+__global__ __launch_bounds__(kThreads, 8) void k_trav_v32(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32) {
+    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (p.tile_order) wave = p.tile_order[wave];
+    trav_v<CAGR_PREC_FP32C, VOP, QM, CONT, false>(p, topo, nodes_, src_, pts64, pts32, wave * 64,
+                                                  p.nq, p.root, p.stop);
+}
+
+template <int K, int QM>
+__global__ __launch_bounds__(kThreads, 8) void k_trav_s32(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32) {
+    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (p.tile_order) wave = p.tile_order[wave];
+    trav_s<CAGR_PREC_FP32C, K, QM, false>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq,
+                                          p.root, p.stop);
+}
+
 // Persistent form: a grid sized to the device pulls wave items {root, stop,
 // q0, qn} (a cloud's tree and up to 64 of its sorted points) from atomic
 // queues until all are empty.  The pull order is cut into one chunk per XCD
@@ -816,6 +842,14 @@ __global__ void k_reset_counter(int32_t* c) {
 // ---------------------------------------------------------------- dispatch
 template <int P, int VOP, int QM, bool CNT>
 int launch_v(const TP& tp, cagr_tree* t, int continuous, int64_t blocks, hipStream_t st) {
+    if constexpr (P == CAGR_PREC_FP32C && !CNT && QM == QM_SELF) {
+        if (!continuous) {
+            k_trav_v32<VOP, QM, false><<<blocks, 64 * t->grid_waves, 0, st>>>(
+                tp, t->topo, t->nodes, t->src, t->pts64, t->pts32);
+            CAGR_LAUNCHED();
+            return CAGR_OK;
+        }
+    }
     if (continuous)
         k_trav_v<P, VOP, QM, true, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(
             tp, t->topo, t->nodes, t->src, t->pts64, t->pts32);
@@ -839,8 +873,12 @@ int launch_vop(const TP& tp, cagr_tree* t, int vop, int continuous, int64_t bloc
 
 template <int P, int K, int QM, bool CNT>
 int launch_s(const TP& tp, cagr_tree* t, int64_t blocks, hipStream_t st) {
-    k_trav_s<P, K, QM, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes, t->src,
-                                                                   t->pts64, t->pts32);
+    if constexpr (P == CAGR_PREC_FP32C && !CNT && QM == QM_SELF)
+        k_trav_s32<K, QM><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes, t->src,
+                                                                 t->pts64, t->pts32);
+    else
+        k_trav_s<P, K, QM, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes,
+                                                                       t->src, t->pts64, t->pts32);
     CAGR_LAUNCHED();
     return CAGR_OK;
 }
