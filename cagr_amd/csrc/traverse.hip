@@ -283,7 +283,8 @@ __device__ __forceinline__ void trav_v(
                 }
             }
             const bool nearl = act && !far;
-            if (topo_leaf(t)) {
+            const bool leaf = topo_leaf(t);
+            if (leaf) {
                 if (nearl) resume = t.escape;
                 if constexpr (CNT) {
                     if (nearl) nnear += t.end - t.begin;
@@ -339,10 +340,10 @@ __device__ __forceinline__ void trav_v(
                         acc.flush_leaf();
                     }
                 }
-                n = t.escape;
-            } else {
-                n = __ballot(nearl) != 0 ? n + 1 : t.escape;
             }
+            // one latch: past a leaf or a subtree no lane enters, the escape;
+            // else the first child
+            n = (leaf || __ballot(nearl) == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             acc.flush_leaf();
@@ -367,7 +368,8 @@ __device__ __forceinline__ void trav_v(
                 else acc.add(d, w, nr.m, p.an);
             }
             const bool nearl = act && !far;
-            if (topo_leaf(t)) {
+            const bool leaf = topo_leaf(t);
+            if (leaf) {
                 if (nearl) resume = t.escape;
                 if constexpr (CNT) {
                     if (nearl) nnear += t.end - t.begin;
@@ -409,10 +411,10 @@ __device__ __forceinline__ void trav_v(
                         }
                     }
                 }
-                n = t.escape;
-            } else {
-                n = __ballot(nearl) != 0 ? n + 1 : t.escape;
             }
+            // one latch: past a leaf or a subtree no lane enters, the escape;
+            // else the first child
+            n = (leaf || __ballot(nearl) == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             if (valid) {
@@ -554,7 +556,8 @@ __device__ __forceinline__ void trav_s(
                 nearl[k] = act && !far;
                 any_near |= nearl[k];
             }
-            if (topo_leaf(t)) {
+            const bool leaf = topo_leaf(t);
+            if (leaf) {
 #pragma unroll
                 for (int k = 0; k < K; ++k)
                     if (nearl[k]) resume[k] = t.escape;
@@ -627,10 +630,10 @@ __device__ __forceinline__ void trav_s(
                         }
                     }
                 }
-                n = t.escape;
-            } else {
-                n = __ballot(any_near) != 0 ? n + 1 : t.escape;
             }
+            // one latch: past a leaf or a subtree no lane enters, the escape;
+            // else the first child
+            n = (leaf || __ballot(any_near) == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             if (valid)
@@ -675,7 +678,8 @@ __device__ __forceinline__ void trav_s(
                 nearl[k] = act && !far;
                 any_near |= nearl[k];
             }
-            if (topo_leaf(t)) {
+            const bool leaf = topo_leaf(t);
+            if (leaf) {
                 for (int k = 0; k < K; ++k)
                     if (nearl[k]) resume[k] = t.escape;
                 if constexpr (CNT) {
@@ -712,10 +716,10 @@ __device__ __forceinline__ void trav_s(
                         }
                     }
                 }
-                n = t.escape;
-            } else {
-                n = __ballot(any_near) != 0 ? n + 1 : t.escape;
             }
+            // one latch: past a leaf or a subtree no lane enters, the escape;
+            // else the first child
+            n = (leaf || __ballot(any_near) == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             if (valid)
