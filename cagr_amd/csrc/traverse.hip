@@ -516,14 +516,20 @@ __device__ __forceinline__ void trav_s(
             }
         };
         for (int32_t n = root; n < stop;) {
+            // internal nodes until a leaf or the end of the tree (this inner
+            // loop leaves the leaf-only state alone, so the register allocator
+            // keeps its loop-carried values in place), then the leaf
+            TopoRec t;
+            bool nearl[K];
+            bool any_near;
+            for (;;) {
             n = __builtin_amdgcn_readfirstlane(n);
-            const TopoRec t = rec32(topo, n);
+            t = rec32(topo, n);
             NodeS32 nrs[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) nrs[k] = rec32(nodes, n * K + k);
             const bool safe = topo_depth(t) <= dsafe;
-            bool nearl[K];
-            bool any_near = false;
+            any_near = false;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const NodeS32& nr = nrs[k];
@@ -555,6 +561,10 @@ __device__ __forceinline__ void trav_s(
                 }
                 nearl[k] = act && !far;
                 any_near |= nearl[k];
+            }
+            if (topo_leaf(t)) break;
+            n = __ballot(any_near) == 0 ? t.escape : n + 1;
+            if (n >= stop) break;
             }
             const bool leaf = topo_leaf(t);
             if (leaf) {
@@ -630,10 +640,8 @@ __device__ __forceinline__ void trav_s(
                         }
                     }
                 }
+                n = t.escape;
             }
-            // one latch: past a leaf or a subtree no lane enters, the escape;
-            // else the first child
-            n = (leaf || __ballot(any_near) == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             if (valid)
@@ -660,10 +668,15 @@ __device__ __forceinline__ void trav_s(
         };
         auto add = [&](const double* d, double sv, int dir) { add_to(acc, d, sv, dir, w); };
         for (int32_t n = root; n < stop;) {
-            n = __builtin_amdgcn_readfirstlane(n);
-            const TopoRec t = rec32(topo, n);
+            // internal nodes until a leaf or the end of the tree, then the leaf
+            // (see the FP32c walk)
+            TopoRec t;
             bool nearl[K];
-            bool any_near = false;
+            bool any_near;
+            for (;;) {
+            n = __builtin_amdgcn_readfirstlane(n);
+            t = rec32(topo, n);
+            any_near = false;
             for (int k = 0; k < K; ++k) {
                 const NodeS64 nr = rec32(nodes, n * K + k);
                 const bool act = resume[k] <= n;
@@ -677,6 +690,10 @@ __device__ __forceinline__ void trav_s(
                 }
                 nearl[k] = act && !far;
                 any_near |= nearl[k];
+            }
+            if (topo_leaf(t)) break;
+            n = __ballot(any_near) == 0 ? t.escape : n + 1;
+            if (n >= stop) break;
             }
             const bool leaf = topo_leaf(t);
             if (leaf) {
@@ -716,10 +733,8 @@ __device__ __forceinline__ void trav_s(
                         }
                     }
                 }
+                n = t.escape;
             }
-            // one latch: past a leaf or a subtree no lane enters, the escape;
-            // else the first child
-            n = (leaf || __ballot(any_near) == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             if (valid)
