@@ -318,3 +318,32 @@ def test_orient_end_to_end_reproduces_bunny(tmp_path, precision):
     assert np.median(cos) > 0.9999 and (cos > 0).mean() == 1.0 and abs(np.median(mag) - 1) < 0.02
     pts, nrm2 = pcio.load_points(os.path.join(out_dir, "bunny.xyz"))
     assert np.array_equal(pts, d["points"]) and np.array_equal(nrm2, nrm.astype(np.float64))
+
+
+@pytest.mark.parametrize("leaf", [16, 32, 64])
+def test_dense_cloud_small_width_matches_oracle(leaf):
+    """A cloud as dense as the benchmark's (80k points on ellipsoids) with its
+    width 0.002: most near leaves then have few near lanes and need no cutoff
+    test, the case where the walks spread a leaf's points over idle lanes.
+    Rows of the fused A2 and A2T_sum at 3000 sampled points against the oracle
+    evaluated at those points (the oracle's queries are plain points, so this
+    is the same sum)."""
+    say("dense", leaf)
+    p = synth.normalize(synth.ellipsoids(80000, 7)).astype(np.float32).astype(np.float64)
+    n = p.shape[0]
+    rng = np.random.default_rng(leaf)
+    idx = np.sort(rng.choice(n, 3000, replace=False))
+    mu = synth.random_normals(n, 3)
+    s3 = rng.normal(size=(n, 3))
+    w = np.full(n, 0.002)
+    wt = WindingNumberTreecode(t(p, torch.float32), leaf_size=leaf)
+    ot = oracle.build_tree(p, 15, leaf)
+    ref_a = np.concatenate([oracle.apply(f"A2{k}", ot, p, mu, w[idx], queries=p[idx])
+                            for k in (1, 2, 3)], 1)
+    ref_t = sum(oracle.apply(f"A2T{k + 1}", ot, p, s3[:, k:k + 1], w[idx], queries=p[idx])
+                for k in range(3))
+    for prec, tol in (("fp64", TOL64), ("fp32c", TOL32C)):
+        a = wt.forward_A2(t(mu), t(w), precision=prec).cpu().numpy()[idx]
+        r = wt.forward_A2T_sum(t(s3), t(w), precision=prec).cpu().numpy()[idx]
+        assert normwise(a, ref_a) <= tol, (prec, normwise(a, ref_a))
+        assert normwise(r, ref_t) <= tol, (prec, normwise(r, ref_t))
