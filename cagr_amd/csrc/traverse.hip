@@ -854,6 +854,30 @@ __global__ __launch_bounds__(512) void k_trav_s_items(
                                    it.y);
 }
 
+// the FP32c persistent walks held to 64 VGPRs: the default grid is sized
+// for 8 waves per SIMD (as the regular-grid FP32c walks)
+template <int VOP>
+__global__ __launch_bounds__(512, 8) void k_trav_v_items32(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32, Items q) {
+    int4 it;
+    while (next_item(q, it))
+        trav_v<CAGR_PREC_FP32C, VOP, QM_SELF, false, false>(p, topo, nodes_, src_, pts64, pts32,
+                                                           it.z, it.z + it.w, it.x, it.y);
+}
+
+template <int K>
+__global__ __launch_bounds__(512, 8) void k_trav_s_items32(
+    TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
+    const void* __restrict__ src_, const double* __restrict__ pts64,
+    const float* __restrict__ pts32, Items q) {
+    int4 it;
+    while (next_item(q, it))
+        trav_s<CAGR_PREC_FP32C, K, QM_SELF, false>(p, topo, nodes_, src_, pts64, pts32, it.z,
+                                                   it.z + it.w, it.x, it.y);
+}
+
 __global__ void k_reset_counter(int32_t* c) {
     if (threadIdx.x < kQueues) c[threadIdx.x * kCounterStride] = 0;
 }
@@ -932,6 +956,8 @@ int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t
         case CAGR_OP_A2:
             if (continuous)
                 k_trav_v_items<P, VOP_A2, true, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            else if constexpr (P == CAGR_PREC_FP32C && !CNT)
+                k_trav_v_items32<VOP_A2><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             else
                 k_trav_v_items<P, VOP_A2, false, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             break;
@@ -948,7 +974,10 @@ int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t
             k_trav_s_items<P, 1, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             break;
         case CAGR_OP_A2T_SUM:
-            k_trav_s_items<P, 3, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            if constexpr (P == CAGR_PREC_FP32C && !CNT)
+                k_trav_s_items32<3><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
+            else
+                k_trav_s_items<P, 3, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             break;
         default:
             set_error("unknown op");
