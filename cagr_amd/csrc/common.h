@@ -54,8 +54,8 @@ template <int K> struct alignas(16) SrcS64 { double y[3]; double s[K]; };
 template <int K> struct alignas(16) SrcS32 { float y[3]; float a[K]; };
 //   a[k] = s_k * exp(-An*y_{dir_k}/2)   (= s_k for AT)
 struct alignas(16) NodeS64 { double c[3]; double s; };
-struct alignas(16) NodeS32 { double c[3]; float s; float e; };
-//   e = exp(-An*c_{dir}/2) at this set's centre
+struct alignas(16) NodeS32 { double c[3]; float se; float pad; };
+//   se = s * exp(-An*c_{dir}/2) at this set's centre (the product in FP64)
 
 // persistent traversal queues: one per XCD, counters a cache line apart
 constexpr int kQueues = 8;

@@ -548,9 +548,9 @@ __device__ __forceinline__ void trav_s(
                         bool in = false;   // far at depth <= dsafe proves r > w
                         if (!safe) in = inside_fp32(r2, w2, e0, e1, e2, w);
                         const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
-                        const float g = eq[k] * nr.e * nr.s;
-                        const float gi3 = g * (ir * ir * ir);
-                        const float hg = hanf * g * ir;
+                        const float gi = eq[k] * nr.se * ir;   // g / r
+                        const float hg = hanf * gi;
+                        const float gi3 = gi * (ir * ir);      // g / r^3
                         for (int cc = 0; cc < 3; ++cc) fp[cc] = fmaf(gi3, d[cc], fp[cc]);
                         if constexpr (K == 3) {
                             fp[k] += hg;

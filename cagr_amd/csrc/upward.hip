@@ -106,8 +106,8 @@ struct Rec<1, K, P> {
         } else {
             NodeS32 r;
             for (int d = 0; d < 3; ++d) r.c[d] = c[d];
-            r.s = (float)mm[0];
-            r.e = (float)exp(-0.5 * an * pick3(c, dir0 + k));
+            r.se = (float)(mm[0] * exp(-0.5 * an * pick3(c, dir0 + k)));
+            r.pad = 0.f;
             ((NodeS32*)nodes)[n * K + k] = r;
         }
     }
