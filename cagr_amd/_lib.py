@@ -33,7 +33,8 @@ EXPORTS = (
     "cagr_apply_weighted", "cagr_apply_adjoint", "cagr_apply_queries", "cagr_grid_query",
     "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
     "cagr_order_items", "cagr_profile", "cagr_profile_read", "cagr_adjoint_plan",
-    "cagr_apply_adjoint_planned",
+    "cagr_apply_adjoint_planned", "cagr_pgr_build", "cagr_pgr_destroy", "cagr_pgr_info",
+    "cagr_pgr_read",
 )
 
 _P, _I, _D, _C = ct.c_void_p, ct.c_int64, ct.c_double, ct.c_int
@@ -60,6 +61,10 @@ _SIGS = {
     "cagr_profile_read": (_C, [_P, _C, _P, _P, _P, _I]),
     "cagr_adjoint_plan": (_C, [_P, _P, _C, _P]),
     "cagr_apply_adjoint_planned": (_C, [_P, _C, _C, _P, _C, _P, _C, _D, _P, _P]),
+    "cagr_pgr_build": (_C, [_P, _I, _C, _C, _P, ct.POINTER(_P)]),
+    "cagr_pgr_destroy": (None, [_P]),
+    "cagr_pgr_info": (_C, [_P, _P, _P, _P, _P]),
+    "cagr_pgr_read": (_C, [_P, _P, _P, _P, _P]),
 }
 
 _lib = None

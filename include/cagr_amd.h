@@ -14,6 +14,7 @@
  *   wnt/wn_treecode_cuda/wn_treecode_cuda_torch_interface.cu:41-134  scatter_point_attrs_to_nodes
  *   wnt/wn_treecode_cuda/wn_treecode_cuda_torch_interface.cu:137-777 multiply_by_{A,AT,A3,G,A21,A22,A23,A2T1,A2T2,A2T3}
  *   CUDA-based/ext/gaussrecon_src/main_GaussRecon_cuda.cu:427-466     multiply_by_A3 grid query
+ *   CUDA-based/ext/gaussrecon_src/Octree.cpp:77-400                   Octree::setTree (PGR query set)
  */
 #ifndef CAGR_AMD_H
 #define CAGR_AMD_H
@@ -202,6 +203,31 @@ int cagr_order_items(cagr_tree* tree, const int64_t* query_cost);
 int cagr_profile(cagr_tree* tree, int enable);
 int cagr_profile_read(cagr_tree* tree, int kind, int64_t* launches, double* total_ms,
                       double* each_ms, int64_t cap);
+
+/* GaussRecon's PGR adaptive octree over oriented samples and its query
+ * corners: replaces Octree::setTree(file, d, d_min) and the gridDataVector it
+ * leaves (gaussrecon_src/Octree.cpp:77-400, used at
+ * main_GaussRecon_cuda.cu:255-291).  points: [n,3] float32, the samples as
+ * the reference reads them from its .xyz file (raw coordinates; the normals
+ * take no part in the octree).  max_depth = the reference's -d (default 10),
+ * in [3, 13]; min_depth = -m (default 1), in [0, max_depth].  Synchronises the
+ * stream (sizes are known only after the build). */
+typedef struct cagr_pgr cagr_pgr;
+int cagr_pgr_build(const float* points, int64_t n, int max_depth, int min_depth, void* stream,
+                   cagr_pgr** out);
+void cagr_pgr_destroy(cagr_pgr* pgr);
+/* sizes (HOST): leaves, corners (= gridDataVector.size()), depth (=
+ * Octree::maxDepth after the build: tree height + 1); scale_center[4] =
+ * Octree::maxScale and the box origin used for the [0,1]^3 mapping.  Any
+ * pointer may be NULL. */
+int cagr_pgr_info(const cagr_pgr* pgr, int64_t* n_leaves, int64_t* n_corners, int* depth,
+                  float* scale_center);
+/* device copies: samples [n,3] in [0,1]^3 (Octree::samplePoints after the
+ * mapping, Octree.cpp:256-259); corners [n_corners,3] in [0,1]^3 in
+ * gridDataVector order (GridData::coords); leaves [n_leaves,4] int32
+ * (depth, x, y, z offsets) in preorder.  Any pointer may be NULL. */
+int cagr_pgr_read(const cagr_pgr* pgr, float* samples, float* corners, int32_t* leaves,
+                  void* stream);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libcagr_oracle.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libcagr_ref.so")
+_PGR_REF_PATH = os.path.join(_HERE, "_ref", "libpgr_ref.so")
 
 # op codes shared with cagr_oracle.cpp
 OPS = {
@@ -68,6 +69,8 @@ def lib():
         L.orc_adjoint.argtypes = [_I, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P,
                                   _D, _I, _D, _P, _P]
         L.orc_num_threads.restype = _I
+        L.pgr_oracle_build.argtypes = [_P, _I, ct.c_int, ct.c_int, _P, _P, _P, _P]
+        L.pgr_oracle_fetch.argtypes = [_P, _P]
         _lib = L
     return _lib
 
@@ -287,3 +290,61 @@ def adjoint(op: str, tree: Tree, points, g, widths, node_reppoints, queries=None
                       _ptr(tree.node2point_indexstart), _ptr(nr), 2.0 / theta,
                       int(continuous), an, _ptr(gg), _ptr(res))
     return res
+
+
+# ---------------------------------------------------------------- PGR octree
+@dataclass
+class PGR:
+    """GaussRecon's PGR octree query set: samples in [0,1]^3 [N,3] f32, leaves
+    [L,4] i32 (depth, x, y, z) in preorder, corners [G,3] f32 in gridDataVector
+    order, depth = Octree::maxDepth after the build (tree height + 1)."""
+    samples: np.ndarray
+    leaves: np.ndarray
+    corners: np.ndarray
+    depth: int
+
+
+def pgr_build(points, max_depth: int = 10, min_depth: int = 1) -> PGR:
+    """The oracle's restatement (oracle/pgr_oracle.cpp) on float32 samples."""
+    L = lib()
+    p = np.ascontiguousarray(points, dtype=np.float32)
+    s = np.zeros_like(p)
+    nl, nc, md = ct.c_int64(), ct.c_int64(), ct.c_int()
+    rc = L.pgr_oracle_build(_ptr(p), p.shape[0], int(max_depth), int(min_depth), _ptr(s),
+                            ct.byref(nl), ct.byref(nc), ct.byref(md))
+    if rc:
+        raise ValueError("pgr_oracle_build rejected its arguments")
+    lv = np.zeros((nl.value, 4), np.int32)
+    cs = np.zeros((nc.value, 3), np.float32)
+    L.pgr_oracle_fetch(_ptr(lv), _ptr(cs))
+    return PGR(s, lv, cs, md.value)
+
+
+def pgr_ref_available() -> bool:
+    return os.path.exists(_PGR_REF_PATH)
+
+
+def pgr_ref_build(points, max_depth: int = 10, min_depth: int = 1, workdir: str = None) -> PGR:
+    """The reference's own Octree::setTree (oracle/_ref/libpgr_ref.so), fed a
+    .xyz file whose decimal text round-trips the float32 samples exactly."""
+    import tempfile
+    R = ct.CDLL(_PGR_REF_PATH)
+    R.pgr_ref_build.argtypes = [ct.c_char_p, ct.c_int, ct.c_int, _P, _P, _P, _P]
+    R.pgr_ref_fetch.argtypes = [_P, _P, _P]
+    R.pgr_ref_leaves.argtypes = [_P]
+    p = np.ascontiguousarray(points, dtype=np.float32)
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        path = os.path.join(d, "samples.xyz")
+        np.savetxt(path, np.hstack([p, np.zeros_like(p)]), fmt="%.9g")
+        ns, ng, nl = ct.c_int64(), ct.c_int64(), ct.c_int64()
+        md = ct.c_int()
+        if R.pgr_ref_build(path.encode(), int(max_depth), int(min_depth), ct.byref(ns),
+                           ct.byref(ng), ct.byref(md), ct.byref(nl)):
+            raise RuntimeError("reference Octree::setTree failed")
+    s = np.zeros((ns.value, 3), np.float32)
+    g = np.zeros((ng.value, 3), np.float32)
+    sc = np.zeros(4, np.float32)
+    R.pgr_ref_fetch(_ptr(s), _ptr(g), _ptr(sc))
+    lv = np.zeros((nl.value, 4), np.int32)
+    R.pgr_ref_leaves(_ptr(lv))
+    return PGR(s, lv, g, md.value)

@@ -11,6 +11,13 @@ has the reference tree; the fixtures are committed, the reference is not).
 * bunny.npz -- the reference-held end-to-end fixture: the 5000 input points
   of CUDA-based/input/bunny.xyz and the normals main_CAGR.py produced for
   them, CUDA-based/results/bunny.xyz (columns 3..5).
+* pgr.npz -- GaussRecon's PGR octree from the reference's OWN Octree::setTree
+  (CUDA-based/ext/gaussrecon_src/*.cpp with its vendored ANN sources, compiled
+  into oracle/_ref/libpgr_ref.so by oracle/Makefile, driven by
+  oracle/pgr_ref_harness.cpp) for the float32 sample sets of PGR_CASES: the
+  inputs, the counts, SHA-256 digests of the normalised samples, the leaves
+  (depth, offsets in preorder) and the corners (gridDataVector order), and
+  the full arrays for the smallest case.
 
 Usage:  python tests/golden/make_golden.py [--ref /root/reference]
 """
@@ -71,6 +78,41 @@ def ref_core_vectors():
     return out
 
 
+def pgr_inputs(bunny_points):
+    r = np.random.default_rng
+    plane = r(24).uniform(-1, 1, (1500, 3))
+    plane[:, 2] = 0.25
+    return {
+        "bunny_d10_m1": (bunny_points, 10, 1),
+        "bunny_d7_m3": (bunny_points, 7, 3),
+        "sphere_d9_m1": (synth.sphere(4000, 21), 9, 1),
+        "uniform_d8_m2": (r(22).uniform(-1, 1, (2000, 3)), 8, 2),
+        "dup_d8_m1": (np.repeat(r(23).uniform(-0.5, 0.5, (100, 3)), 5, 0), 8, 1),
+        "plane_d9_m1": (plane, 9, 1),
+    }
+
+
+def digest(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def pgr_vectors(bunny_points):
+    out = {}
+    for name, (pts, d, m) in pgr_inputs(bunny_points).items():
+        p = np.asarray(pts, np.float32)
+        g = oracle.pgr_ref_build(p, d, m)
+        if not name.startswith("bunny"):   # the bunny samples are bunny.npz's points
+            out[f"{name}/points"] = p
+        out[f"{name}/args"] = np.array([d, m, g.depth, len(g.leaves), len(g.corners)], np.int64)
+        for field in ("samples", "leaves", "corners"):
+            out[f"{name}/{field}_sha256"] = np.array(digest(getattr(g, field)))
+        if len(g.leaves) < 12500:
+            out[f"{name}/leaves"] = g.leaves
+            out[f"{name}/corners"] = g.corners
+    return out
+
+
 def bunny(ref_root):
     base = os.path.join(ref_root, "CUDA-based")
     pin = np.loadtxt(os.path.join(base, "input", "bunny.xyz"))
@@ -87,8 +129,12 @@ def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libcagr_ref.so is missing (make -C oracle needs the reference tree)")
     np.savez_compressed(os.path.join(HERE, "ref_core.npz"), **ref_core_vectors())
-    np.savez_compressed(os.path.join(HERE, "bunny.npz"), **bunny(args.ref))
-    for f in ("ref_core.npz", "bunny.npz"):
+    bun = bunny(args.ref)
+    np.savez_compressed(os.path.join(HERE, "bunny.npz"), **bun)
+    if not oracle.pgr_ref_available():
+        sys.exit("oracle/_ref/libpgr_ref.so is missing (make -C oracle needs the reference tree)")
+    np.savez_compressed(os.path.join(HERE, "pgr.npz"), **pgr_vectors(bun["points"]))
+    for f in ("ref_core.npz", "bunny.npz", "pgr.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

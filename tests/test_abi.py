@@ -49,10 +49,25 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert L.cagr_tree_build(None, 10, 0, 15, 1, 0.5, None, ct.byref(h)) == 1
     assert b"null points" in L.cagr_last_error()
     assert L.cagr_apply(None, 0, 0, None, 0, None, 0, 1.0, None, None) == 1
+    # PGR octree: depth range, min depth, sample count, null points
+    fake = ct.c_void_p(16)
+    assert L.cagr_pgr_build(fake, 10, 2, 1, None, ct.byref(h)) == 1
+    assert b"max_depth" in L.cagr_last_error()
+    assert L.cagr_pgr_build(fake, 10, 14, 1, None, ct.byref(h)) == 1
+    assert b"max_depth" in L.cagr_last_error()
+    assert L.cagr_pgr_build(fake, 10, 8, 9, None, ct.byref(h)) == 1
+    assert b"min_depth" in L.cagr_last_error()
+    assert L.cagr_pgr_build(fake, 0, 8, 1, None, ct.byref(h)) == 1
+    assert b"number of samples" in L.cagr_last_error()
+    assert L.cagr_pgr_build(None, 10, 8, 1, None, ct.byref(h)) == 1
+    assert b"null points" in L.cagr_last_error()
 
 
 def test_no_cpu_fallback():
     import torch
     from cagr_amd import WindingNumberTreecode
+    from cagr_amd.pgr import PGROctree
     with pytest.raises(RuntimeError):
         WindingNumberTreecode(torch.zeros(10, 3))
+    with pytest.raises(ValueError):
+        PGROctree(torch.zeros(10, 3))
