@@ -34,7 +34,10 @@ schedule on the FP64 oracle on the host cores and on the GPU, 128^3 grid
 query), c2 (100k torus with 5 % outliers: FP64 drop-in matvec pairs, 256^3
 grid query), c4 (500k displaced sphere: autograd forward + backward, FP64
 finite-difference check on 64 entries), c5 (64 clouds, ~3M points: autotuned
-batched persistent launch against one launch per cloud).
+batched persistent launch against one launch per cloud), pgr (GaussRecon on
+the 1M-point ellipsoid cloud: the PGR octree query set built on the GPU
+against the reference's own Octree::setTree on a host core, then the A3
+winding numbers at its corners and samples).
 """
 from __future__ import annotations
 
@@ -73,7 +76,7 @@ def parse():
     ap.add_argument("--solve-tol", type=float, default=1e-3,
                     help="FP64-verified relative residual of the solve comparison")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "pgr"],
                     help="c3 (default): the headline workload; others: see the module docstring")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times and exit (calibrates "
@@ -125,7 +128,8 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.config != "c3":
-        line = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5}[args.config](args)
+        line = {"c1": config_c1, "c2": config_c2, "c4": config_c4, "c5": config_c5,
+                "pgr": config_pgr}[args.config](args)
         print(json.dumps(line), flush=True)
         return
     n = args.points
@@ -609,6 +613,77 @@ def config_c5(args):
             "speedup_tuned_vs_per_cloud": round(tuned / per_cloud, 3),
             "table": [[r.leaf_size, r.theta, r.waves_per_block, round(r.ms_per_pair, 4), r.err]
                       for r in rep.table]}
+
+
+def config_pgr(args):
+    """GaussRecon (main_GaussRecon_cuda.cu:255-486 before marching cubes) on
+    the 1M-point ellipsoid cloud, -d 10 -m 1 -w 0.01: the PGR octree query set
+    on the GPU (cagr_pgr_build) against the reference's own Octree::setTree
+    (oracle/_ref, one host core, as the reference runs it; the port when the
+    reference library is absent), then -wn at every corner and at the samples
+    with the leaf-1 treecode (FP32c and FP64)."""
+    import torch
+    import oracle
+    from cagr_amd import WindingNumberTreecode, synth
+    from cagr_amd.pgr import PGROctree
+    from cagr_amd.query import sample_attributes
+    dev = torch.device("cuda:0")
+    n, D, M, width = args.points, 10, 1, 0.01
+    pts = synth.normalize(synth.ellipsoids(n, args.seed)).astype(np.float32)
+    nml = synth.random_normals(n, args.seed + 1).astype(np.float32)
+    pt = torch.from_numpy(pts).to(dev)
+    PGROctree(pt, D, M)
+    builds = []
+    for _ in range(3):
+        sec, oct_ = _sync_time(lambda: PGROctree(pt, D, M))
+        builds.append(sec)
+    build_s = float(np.median(builds))
+    nc, nl = oct_.corners.shape[0], oct_.leaves.shape[0]
+    log(f"pgr: GPU build {build_s * 1e3:.1f} ms, {nl} leaves, {nc} corners, depth {oct_.depth}")
+    out = {"metric": "PGR query-set build seconds; A3 values at its corners (queries/s)",
+           "config": {"workload": f"GaussRecon on {n} ellipsoid points, -d {D} -m {M} -w {width}, "
+                                  "leaf-1 treecode, theta 0.5", "points": n},
+           "data": "synthetic",
+           "pgr": {"build_s": round(build_s, 4), "leaves": nl, "corners": nc,
+                   "depth": oct_.depth}}
+    p11 = oct_.sample_points()
+    mu = sample_attributes(torch.from_numpy(nml).to(dev))
+    q = oct_.query_points()
+    for prec, dt in (("fp32c", torch.float32), ("fp64", torch.float64)):
+        tb, wt = _sync_time(lambda: WindingNumberTreecode(p11.to(dt), leaf_size=1, theta=0.5,
+                                                          precision=prec))
+        m = mu.to(dt)
+        wt.query(q.to(dt), m, width, precision=prec)
+        qs, vals = _sync_time(lambda: wt.query(q.to(dt), m, width, precision=prec))
+        w = torch.full((n,), width, dtype=dt, device=dev)
+        ss, wn_s = _sync_time(lambda: wt.apply("A3", m, w, continuous=True, precision=prec))
+        iso = -float(torch.kthvalue(wn_s[:, 0].double().cpu(), n // 2 + 1).values)
+        out[prec] = {"tree_build_s": round(tb, 4), "corner_query_s": round(qs, 4),
+                     "corner_queries_per_s": round(nc / qs, 1), "sample_query_s": round(ss, 4),
+                     "isovalue": iso}
+        if prec == "fp32c":
+            v32 = vals.double()
+        else:
+            out["fp32c"]["corner_err_vs_fp64"] = float((v32 - vals.double()).abs().max()
+                                                       / vals.double().abs().max())
+        log(f"pgr {prec}: corners {qs * 1e3:.1f} ms, samples {ss * 1e3:.1f} ms, iso {iso:.4g}")
+    if not args.no_cpu:
+        kind = "reference" if oracle.pgr_ref_available() else "port"
+        fn = oracle.pgr_ref_build if kind == "reference" else oracle.pgr_build
+        t = time.perf_counter()
+        ref = fn(pts, D, M)
+        cpu_s = time.perf_counter() - t
+        same = bool(np.array_equal(ref.corners, oct_.corners.cpu().numpy())
+                    and np.array_equal(ref.leaves, oct_.leaves.cpu().numpy()))
+        log(f"pgr: CPU {kind} build {cpu_s:.2f} s, identical query set: {same}")
+        out["cpu_baseline"] = {"value": round(cpu_s, 3), "unit": "s (PGR build)", "cores": 1,
+                               "kind": kind,
+                               "sample": "the whole build on the same 1M points"
+                                         + (" (incl. writing and parsing its .xyz file)"
+                                            if kind == "reference" else "")}
+        out["pgr"]["identical_to_cpu"] = same
+        out["pgr"]["speedup_vs_cpu"] = round(cpu_s / build_s, 1)
+    return out
 
 
 if __name__ == "__main__":

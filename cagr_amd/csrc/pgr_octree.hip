@@ -78,15 +78,13 @@ __device__ inline uint64_t tkey(int d, uint32_t x, uint32_t y, uint32_t z) {
     return ((uint64_t)z << (2 * d)) | ((uint64_t)y << d) | x;
 }
 
-// cell index of coordinate p at depth d: descend comparing with the exact
-// cell centres, p > centre going up (Cube::CornerIndex, Cube.cpp:46-52)
+// cell index of coordinate p in (0, 1] at depth d.  The reference descends
+// comparing p with the exact cell centres, going up iff p > centre
+// (Cube::CornerIndex, Cube.cpp:46-52); with t = p * 2^(l+1) (exact) the step
+// is o' = 2o + (t > 2o + 1), whose closed form is o_d = ceil(p * 2^d) - 1.
 __device__ inline uint32_t descend(float p, int d) {
-    uint32_t o = 0;
-    for (int l = 0; l < d; ++l) {
-        const float c = (float)((0.5 + (double)o) / (double)(1 << l));
-        o = 2 * o + (p > c ? 1u : 0u);
-    }
-    return o;
+    const int c = (int)ceilf(ldexpf(p, d)) - 1;
+    return (uint32_t)max(c, 0);
 }
 
 // float centre of cell o at depth d (ReconOctNode::centerAndWidth)
@@ -95,21 +93,23 @@ __device__ inline float centre(int d, uint32_t o) {
     return (float)((0.5 + (double)o) * w);
 }
 
-// B-spline weights of the 3 cells along one axis (Octree.cpp:1302-1307)
-__device__ inline void spline3(float c, float p, double width, double out[3]) {
+// B-spline weights of the 3 cells along one axis (Octree.cpp:1302-1307); the
+// divisions by the width 2^-d are multiplications by 2^d, which round the same
+__device__ inline void spline3(float c, float p, double width, double inv_width, double out[3]) {
     const float cp = c - p;
-    double x = (cp - width) / width;
+    double x = (cp - width) * inv_width;
     out[0] = 1.125 + 1.5 * x + 0.5 * x * x;
-    x = cp / width;
+    x = cp * inv_width;
     out[1] = 0.75 - x * x;
     out[2] = 1.0 - out[1] - out[0];
 }
 
 __device__ inline void axis_weights(const float p[3], int d, uint32_t o[3], double dx[3][3]) {
     const double w = (double)((float)1.0 / (1 << d));
+    const double iw = (double)(1 << d);
     for (int k = 0; k < 3; ++k) {
         o[k] = descend(p[k], d);
-        spline3(centre(d, o[k]), p[k], w, dx[k]);
+        spline3(centre(d, o[k]), p[k], w, iw, dx[k]);
     }
 }
 
@@ -196,20 +196,30 @@ __global__ __launch_bounds__(kT) void k_starts(const uint64_t* __restrict__ keys
 }
 
 // serial float sum of a segment in record order, in the reference's
-// float-accumulator / double-term form; one wave per segment
+// float-accumulator / double-term form; one wave per segment.  The wave
+// evaluates 64 terms at once, then folds them in lane order: three dependent
+// instructions per term (widen, add, narrow) on a wave-uniform loop.
 template <class F>
 __device__ inline float wave_serial_sum(int32_t lo, int32_t hi, F term) {
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
     const int lane = threadIdx.x & 63;
     float acc = 0.f;
+    auto fold = [&](int vlo, int vhi, int l) {
+        const uint64_t b = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(vlo, l) |
+                           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vhi, l) << 32);
+        acc = (float)((double)acc + __longlong_as_double((long long)b));
+    };
     for (int32_t base = lo; base < hi; base += 64) {
         const int32_t r = base + lane;
         const int64_t v = __double_as_longlong(r < hi ? term(r) : 0.0);
         const int vlo = (int)v, vhi = (int)(v >> 32);
-        const int cnt = min(64, hi - base);
-        for (int l = 0; l < cnt; ++l) {
-            const uint64_t b = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(vlo, l) |
-                               ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(vhi, l) << 32);
-            acc = (float)((double)acc + __longlong_as_double((long long)b));
+        if (hi - base >= 64) {
+#pragma unroll
+            for (int l = 0; l < 64; ++l) fold(vlo, vhi, l);
+        } else {
+            const int cnt = hi - base;
+            for (int l = 0; l < cnt; ++l) fold(vlo, vhi, l);
         }
     }
     return acc;
@@ -232,16 +242,35 @@ __global__ __launch_bounds__(kT) void k_cwc_recs(const float* __restrict__ p, in
     vals[r] = (int32_t)r;
 }
 
-__global__ __launch_bounds__(kT) void k_cwc_sum(const float* __restrict__ p, int d,
-                                                const int32_t* __restrict__ vals,
-                                                const int32_t* __restrict__ start, int32_t U,
+// one entry per cell of every depth: its record run [lo, hi) in the
+// concatenated sorted records and its depth
+struct Seg {
+    int64_t lo, hi;
+    int32_t d, pad;
+};
+
+__global__ __launch_bounds__(kT) void k_seg_table(const int32_t* __restrict__ start, int32_t U,
+                                                  int d, int64_t base, Seg* __restrict__ seg) {
+    const int32_t s = (int32_t)((int64_t)blockIdx.x * kT + threadIdx.x);
+    if (s < U) seg[s] = Seg{base + start[s], base + start[s + 1], d, 0};
+}
+
+// all depths in one launch, the shallow (longest, serial) cells first, so
+// their chains run side by side
+__global__ __launch_bounds__(kT) void k_cwc_sum(const float* __restrict__ p,
+                                                const int32_t* __restrict__ vals, int64_t per_depth,
+                                                const Seg* __restrict__ seg, int32_t nseg,
                                                 float* __restrict__ cwc) {
     const int32_t s = (int32_t)(((int64_t)blockIdx.x * kT + threadIdx.x) >> 6);
-    if (s >= U) return;
-    const float acc = wave_serial_sum(start[s], start[s + 1], [&](int32_t r) {
-        const int32_t v = vals[r];
-        const int64_t i = v / 27;
-        const int t = v % 27;
+    if (s >= nseg) return;
+    const Seg g = seg[s];
+    const int d = __builtin_amdgcn_readfirstlane(g.d);
+    const int32_t* v = vals + (int64_t)d * per_depth;
+    const int64_t off = (int64_t)d * per_depth;
+    const float acc = wave_serial_sum((int32_t)(g.lo - off), (int32_t)(g.hi - off), [&](int32_t r) {
+        const int32_t rv = v[r];
+        const int64_t i = rv / 27;
+        const int t = rv % 27;
         const float q[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
         uint32_t o[3];
         double dx[3][3];
@@ -392,10 +421,18 @@ __global__ __launch_bounds__(kT) void k_compact(const uint64_t* __restrict__ in,
     if (s < U && pos[s + 1] != pos[s]) out[pos[s]] = in[s];
 }
 
-__global__ __launch_bounds__(kT) void k_depth_count(const uint64_t* __restrict__ k, int32_t m,
-                                                    int32_t* __restrict__ cnt) {
-    const int32_t r = (int32_t)((int64_t)blockIdx.x * kT + threadIdx.x);
-    if (r < m) atomicAdd(&cnt[(int)(k[r] >> 39)], 1);
+// first position of each depth in the depth-sorted cell keys: off[d], d < 17
+__global__ __launch_bounds__(64) void k_depth_off(const uint64_t* __restrict__ k, int32_t m,
+                                                  int32_t* __restrict__ off) {
+    const int d = threadIdx.x;
+    if (d > 16) return;
+    int32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if ((int)(k[mid] >> 39) < d) lo = mid + 1;
+        else hi = mid;
+    }
+    off[d] = lo;
 }
 
 // ------------------------------------------------------------ clipped tree
@@ -598,29 +635,48 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
     const float* p = g->samples;
 
     // centre weights per depth 0..S: records (cell, sample*27 + slot), stable
-    // sort by cell, serial sum per cell
+    // sort by cell, kept per depth; then one serial sum per cell of every depth
     Levels L;
     DBuf<uint64_t> ukeys;
     DBuf<uint64_t> lk[kMaxD + 1];
-    DBuf<float> lc[kMaxD + 1];
-    if ((rc = w.reserve(54 * n))) return rc;
+    DBuf<int32_t> allv;
+    DBuf<int32_t> sd[kMaxD + 1];
+    DBuf<Seg> segs;
+    DBuf<float> cw;
+    int64_t seg_off[kMaxD + 2] = {0};
+    const int64_t per_depth = 27 * n;
+    if ((rc = w.reserve(54 * n)) || (rc = allv.ensure((size_t)(S + 1) * per_depth)) ||
+        (rc = ukeys.ensure(per_depth)))
+        return rc;
     for (int d = 0; d <= S; ++d) {
-        const int64_t m = 27 * n;
+        const int64_t m = per_depth;
         k_cwc_recs<<<nblk(m), kT, 0, st>>>(p, n, d, w.keys.p, w.vals.p);
         CAGR_LAUNCHED();
         if ((rc = w.sort(m, 3 * d + 1))) return rc;
-        if ((rc = ukeys.ensure(m))) return rc;
         int32_t U = 0;
         if ((rc = w.runs(m, 1ull << (3 * d), ukeys.p, &U))) return rc;
-        if ((rc = lk[d].ensure(U)) || (rc = lc[d].ensure(U))) return rc;
+        if ((rc = lk[d].ensure(U)) || (rc = sd[d].ensure((size_t)U + 1))) return rc;
         CAGR_HIP(hipMemcpyAsync(lk[d].p, ukeys.p, U * sizeof(uint64_t), hipMemcpyDeviceToDevice,
                                 st));
-        k_cwc_sum<<<nblk(64 * (int64_t)U), kT, 0, st>>>(p, d, w.vals.p, w.start.p, U, lc[d].p);
-        CAGR_LAUNCHED();
+        CAGR_HIP(hipMemcpyAsync(sd[d].p, w.start.p, ((size_t)U + 1) * sizeof(int32_t),
+                                hipMemcpyDeviceToDevice, st));
+        CAGR_HIP(hipMemcpyAsync(allv.p + d * per_depth, w.vals.p, m * sizeof(int32_t),
+                                hipMemcpyDeviceToDevice, st));
+        seg_off[d + 1] = seg_off[d] + U;
         L.uk[d] = lk[d].p;
-        L.cw[d] = lc[d].p;
         L.U[d] = U;
     }
+    const int64_t nseg = seg_off[S + 1];
+    CAGR_REQUIRE(nseg < (int64_t(1) << 31), CAGR_ERR_ARG, "too many cells for 32-bit indices");
+    if ((rc = cw.ensure(nseg)) || (rc = segs.ensure(nseg))) return rc;
+    for (int d = 0; d <= S; ++d) {
+        k_seg_table<<<nblk(L.U[d]), kT, 0, st>>>(sd[d].p, L.U[d], d, d * per_depth,
+                                                 segs.p + seg_off[d]);
+        CAGR_LAUNCHED();
+    }
+    k_cwc_sum<<<nblk(64 * nseg), kT, 0, st>>>(p, allv.p, per_depth, segs.p, (int32_t)nseg, cw.p);
+    CAGR_LAUNCHED();
+    for (int d = 0; d <= S; ++d) L.cw[d] = cw.p + seg_off[d];
     for (int d = S + 1; d <= kMaxD; ++d) {
         L.uk[d] = nullptr;
         L.cw[d] = nullptr;
@@ -657,23 +713,21 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
         CAGR_HIP(hipStreamSynchronize(st));  // before the block's buffers are freed
     }
 
-    // cells with normals per depth; the tree height is the deepest of them
-    int32_t cnt[16] = {0};
+    // cells with normals per depth (the keys are depth-major); the tree height
+    // is the deepest of them
+    int32_t nb_off[17] = {0};
     {
         DBuf<int32_t> c;
-        if ((rc = c.ensure(16))) return rc;
-        CAGR_HIP(hipMemsetAsync(c.p, 0, 16 * sizeof(int32_t), st));
-        if (nnb) {
-            k_depth_count<<<nblk(nnb), kT, 0, st>>>(nb_keys.p, nnb, c.p);
-            CAGR_LAUNCHED();
-        }
-        CAGR_HIP(hipMemcpyAsync(cnt, c.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        if ((rc = c.ensure(17))) return rc;
+        k_depth_off<<<1, 64, 0, st>>>(nb_keys.p, nnb, c.p);
+        CAGR_LAUNCHED();
+        CAGR_HIP(hipMemcpyAsync(nb_off, c.p, sizeof(nb_off), hipMemcpyDeviceToHost, st));
         CAGR_HIP(hipStreamSynchronize(st));
     }
+    int32_t cnt[16];
     int H = 0;
-    int32_t nb_off[17] = {0};
     for (int d = 0; d < 16; ++d) {
-        nb_off[d + 1] = nb_off[d] + cnt[d];
+        cnt[d] = nb_off[d + 1] - nb_off[d];
         if (cnt[d]) H = d;
     }
 
