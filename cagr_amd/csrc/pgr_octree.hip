@@ -568,6 +568,37 @@ struct DBuf {
     }
 };
 
+// a second stream (default priority) for work that overlaps the main one
+struct SideStream {
+    hipStream_t main = nullptr, s = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int create(hipStream_t st) {
+        main = st;
+        CAGR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        CAGR_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        CAGR_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        return CAGR_OK;
+    }
+    // the side stream waits for everything enqueued on the main stream so far
+    hipError_t fork() {
+        hipError_t e = hipEventRecord(ev_fork, main);
+        return e != hipSuccess ? e : hipStreamWaitEvent(s, ev_fork, 0);
+    }
+    // the main stream waits for everything enqueued on the side stream
+    hipError_t join() {
+        hipError_t e = hipEventRecord(ev_join, s);
+        return e != hipSuccess ? e : hipStreamWaitEvent(main, ev_join, 0);
+    }
+    ~SideStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+    }
+};
+
 struct Work {
     DBuf<uint64_t> keys, ktmp;
     DBuf<int32_t> vals, vtmp, flags, start, stmp;
@@ -635,19 +666,43 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
     const float* p = g->samples;
 
     // centre weights per depth 0..S: records (cell, sample*27 + slot), stable
-    // sort by cell, kept per depth; then one serial sum per cell of every depth
+    // sort by cell, kept per depth; then one serial sum per cell.  The shallow
+    // depths hold the long serial chains (the root's has N terms): their sums
+    // start on a second stream as soon as they are sorted and run beside the
+    // sorts of the deeper depths, whose sums follow in one launch.
     Levels L;
     DBuf<uint64_t> ukeys;
     DBuf<uint64_t> lk[kMaxD + 1];
     DBuf<int32_t> allv;
     DBuf<int32_t> sd[kMaxD + 1];
-    DBuf<Seg> segs;
-    DBuf<float> cw;
+    DBuf<Seg> segs[2];
+    DBuf<float> cw[2];
     int64_t seg_off[kMaxD + 2] = {0};
     const int64_t per_depth = 27 * n;
-    if ((rc = w.reserve(54 * n)) || (rc = allv.ensure((size_t)(S + 1) * per_depth)) ||
-        (rc = ukeys.ensure(per_depth)))
+    const int early = std::min(3, S + 1);   // depths summed on the side stream
+    SideStream side;
+    if ((rc = side.create(st)) || (rc = w.reserve(54 * n)) ||
+        (rc = allv.ensure((size_t)(S + 1) * per_depth)) || (rc = ukeys.ensure(per_depth)))
         return rc;
+    // seg table + sums of depths [d0, d1) into group g, on stream s
+    auto sums = [&](int d0, int d1, int g, hipStream_t s_tab, hipStream_t s_sum) -> int {
+        const int64_t cnt = seg_off[d1] - seg_off[d0];
+        int rc2;
+        if ((rc2 = segs[g].ensure(cnt)) || (rc2 = cw[g].ensure(cnt))) return rc2;
+        for (int d = d0; d < d1; ++d) {
+            k_seg_table<<<nblk(L.U[d]), kT, 0, s_tab>>>(sd[d].p, L.U[d], d, d * per_depth,
+                                                        segs[g].p + seg_off[d] - seg_off[d0]);
+            CAGR_LAUNCHED();
+            L.cw[d] = cw[g].p + seg_off[d] - seg_off[d0];
+        }
+        if (s_tab != s_sum) CAGR_HIP(side.fork());
+        if (cnt) {
+            k_cwc_sum<<<nblk(64 * cnt), kT, 0, s_sum>>>(p, allv.p, per_depth, segs[g].p,
+                                                        (int32_t)cnt, cw[g].p);
+            CAGR_LAUNCHED();
+        }
+        return CAGR_OK;
+    };
     for (int d = 0; d <= S; ++d) {
         const int64_t m = per_depth;
         k_cwc_recs<<<nblk(m), kT, 0, st>>>(p, n, d, w.keys.p, w.vals.p);
@@ -663,20 +718,14 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
         CAGR_HIP(hipMemcpyAsync(allv.p + d * per_depth, w.vals.p, m * sizeof(int32_t),
                                 hipMemcpyDeviceToDevice, st));
         seg_off[d + 1] = seg_off[d] + U;
+        CAGR_REQUIRE(seg_off[d + 1] < (int64_t(1) << 31), CAGR_ERR_ARG,
+                     "too many cells for 32-bit indices");
         L.uk[d] = lk[d].p;
         L.U[d] = U;
+        if (d + 1 == early && (rc = sums(0, early, 0, st, side.s))) return rc;
     }
-    const int64_t nseg = seg_off[S + 1];
-    CAGR_REQUIRE(nseg < (int64_t(1) << 31), CAGR_ERR_ARG, "too many cells for 32-bit indices");
-    if ((rc = cw.ensure(nseg)) || (rc = segs.ensure(nseg))) return rc;
-    for (int d = 0; d <= S; ++d) {
-        k_seg_table<<<nblk(L.U[d]), kT, 0, st>>>(sd[d].p, L.U[d], d, d * per_depth,
-                                                 segs.p + seg_off[d]);
-        CAGR_LAUNCHED();
-    }
-    k_cwc_sum<<<nblk(64 * nseg), kT, 0, st>>>(p, allv.p, per_depth, segs.p, (int32_t)nseg, cw.p);
-    CAGR_LAUNCHED();
-    for (int d = 0; d <= S; ++d) L.cw[d] = cw.p + seg_off[d];
+    if ((rc = sums(early, S + 1, 1, st, st))) return rc;
+    CAGR_HIP(side.join());
     for (int d = S + 1; d <= kMaxD; ++d) {
         L.uk[d] = nullptr;
         L.cw[d] = nullptr;
