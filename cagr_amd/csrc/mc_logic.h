@@ -1,0 +1,624 @@
+// Marching cubes on the PGR octree, per leaf (GaussRecon's
+// Octree::GetMCIsoTriangles with fullDepthIso 0, nonLinearFit 1, no
+// barycentres, triangles out: gaussrecon_src/Octree.cpp:518-1257 with
+// MarchingCubes.cpp, Cube.cpp and Geometry.cpp:74-220).
+//
+// Every function is a pure function of the tree view and its arguments,
+// marked CAGR_HD (= __host__ __device__ under hipcc): the kernels of
+// pgr_mesh.hip call them once per leaf, and tests/host_logic/ compiles this
+// header with g++ to replay the mesh serially against the reference's own
+// build before anything runs on a GPU.
+//
+// The reference walks a pointer octree.  Here a node is (depth, offsets) and
+// the tree is the sorted list of its internal nodes:
+//   * a node's marching-cubes index is GetIndex of the values at its own 8
+//     grid corners -- the reference propagates leaf corner bits upwards
+//     (SetIsoSurfaceCorners, :518-611), and the bit of corner c of a node
+//     always comes from the leaf reached by descending into child c, which
+//     holds the same grid point;
+//   * a face or edge neighbour "exists and has children" (the only case the
+//     root search and the face walk act on) iff the same-depth neighbour
+//     cell is an internal node: coarser neighbours have no children by
+//     construction, and ReconOctNode::__edgeNeighbor returns either that
+//     cell, NULL or a childless coarser node.
+#pragma once
+#include <cstdint>
+
+#include "mc_tables.h"
+
+#ifndef CAGR_HD
+#define CAGR_HD __host__ __device__
+#endif
+
+namespace cagr {
+namespace mc {
+
+// limits of the per-leaf work (checked; exceeding them is reported)
+constexpr int kMaxEdges = 512;     // iso edges of one leaf incl. finer neighbours
+constexpr int kMaxVerts = 512;     // distinct roots met by one leaf
+constexpr int kMaxWalk = 64;       // explicit stacks of the descents
+
+struct Node {
+    int d;
+    int32_t o[3];
+};
+
+// the tree and the field, all arrays in the caller's memory space
+struct View {
+    int md;                        // Octree::maxDepth (tree height + 1)
+    const uint64_t* in_keys;       // internal nodes: d<<39 | z<<26 | y<<13 | x, ascending
+    int64_t n_in;
+    const int64_t* ck;             // grid-corner keys (ReconOctNode::getCornerIndex), ascending
+    const int32_t* ck_idx;         // their gridDataVector positions
+    int64_t n_ck;
+    const float* values;           // GridData::value per gridDataVector position
+    float iso;
+    float smooth_width;
+    float grid;                    // 1 / 2^(md + 1): coordinate of one key unit
+};
+
+struct RootInfo {
+    Node node;
+    int edge;
+    int64_t key;
+};
+
+// ------------------------------------------------------------ cube indexing
+// (Cube.cpp: corner c = x | y<<1 | z<<2, edge e = orientation<<2 | i | j<<1,
+// face f = 2*axis + side)
+CAGR_HD inline int corner_bit(int c) {               // MarchingCubes::cornerMap
+    const int m[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    return m[c];
+}
+CAGR_HD inline void edge_corners(int e, int& c1, int& c2) {
+    const int o = e >> 2, i = e & 1, j = (e >> 1) & 1;
+    if (o == 0) { c1 = (i << 1) | (j << 2); c2 = c1 | 1; }
+    else if (o == 1) { c1 = i | (j << 2); c2 = c1 | 2; }
+    else { c1 = i | (j << 1); c2 = c1 | 4; }
+}
+CAGR_HD inline void faces_of_edge(int e, int& f1, int& f2) {     // FacesAdjacentToEdge
+    const int o = e >> 2, i = e & 1, j = (e >> 1) & 1;
+    if (o == 0) { f1 = 2 + i; f2 = 4 + j; }
+    else if (o == 1) { f1 = 0 + i; f2 = 4 + j; }
+    else { f1 = 0 + i; f2 = 2 + j; }
+}
+CAGR_HD inline int face_reflect_edge(int e, int f) {             // FaceReflectEdgeIndex
+    const int axis = f >> 1, o = e >> 2;
+    if (o == axis) return e;
+    // the edge's coordinate along `axis` is i (first free axis) or j (second)
+    const int first = (o == 0) ? 1 : 0;
+    return (axis == first) ? (e ^ 1) : (e ^ 2);
+}
+CAGR_HD inline int edge_reflect_edge(int e) { return e ^ 3; }   // EdgeReflectEdgeIndex
+CAGR_HD inline int face_of_edges(int e1, int e2) {               // FaceAdjacentToEdges
+    int f1, f2, g1, g2;
+    faces_of_edge(e1, f1, f2);
+    faces_of_edge(e2, g1, g2);
+    if (f1 == g1 || f1 == g2) return f1;
+    if (f2 == g1 || f2 == g2) return f2;
+    return -1;
+}
+CAGR_HD inline void face_corners(int f, int c[4]) {              // FaceCorners
+    const int i = f & 1;
+    switch (f >> 1) {
+        case 0: c[0] = i; c[1] = i | 2; c[2] = i | 4; c[3] = i | 6; return;
+        case 1: c[0] = i << 1; c[1] = 1 | (i << 1); c[2] = (i << 1) | 4; c[3] = 1 | (i << 1) | 4; return;
+        default: c[0] = i << 2; c[1] = 1 | (i << 2); c[2] = 2 | (i << 2); c[3] = 3 | (i << 2); return;
+    }
+}
+CAGR_HD inline int edge_mask(int mc) {           // edges whose corners differ in sign
+    int m = 0;
+    for (int e = 0; e < 12; ++e) {
+        int c1, c2;
+        edge_corners(e, c1, c2);
+        if (((mc >> corner_bit(c1)) & 1) != ((mc >> corner_bit(c2)) & 1)) m |= 1 << e;
+    }
+    return m;
+}
+CAGR_HD inline bool has_edge_roots(int mc, int e) { return (edge_mask(mc) >> e) & 1; }
+CAGR_HD inline bool has_roots(int mc) { return mc != 0 && mc != 255; }
+
+// ------------------------------------------------------------ tree lookups
+CAGR_HD inline uint64_t node_key(int d, int64_t x, int64_t y, int64_t z) {
+    return ((uint64_t)d << 39) | ((uint64_t)z << 26) | ((uint64_t)y << 13) | (uint64_t)x;
+}
+template <class K>
+CAGR_HD inline int64_t lower_bound(const K* a, int64_t n, K v) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+CAGR_HD inline bool is_internal(const View& t, int d, int64_t x, int64_t y, int64_t z) {
+    const int64_t lim = (int64_t)1 << d;
+    if (x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) return false;
+    const uint64_t k = node_key(d, x, y, z);
+    const int64_t i = lower_bound(t.in_keys, t.n_in, k);
+    return i < t.n_in && t.in_keys[i] == k;
+}
+CAGR_HD inline bool is_internal(const View& t, const Node& n) {
+    return is_internal(t, n.d, n.o[0], n.o[1], n.o[2]);
+}
+CAGR_HD inline Node child(const Node& n, int c) {
+    return Node{n.d + 1, {2 * n.o[0] + (c & 1), 2 * n.o[1] + ((c >> 1) & 1), 2 * n.o[2] + ((c >> 2) & 1)}};
+}
+CAGR_HD inline int child_index(const Node& n) {
+    return (n.o[0] & 1) | ((n.o[1] & 1) << 1) | ((n.o[2] & 1) << 2);
+}
+CAGR_HD inline Node parent(const Node& n) {
+    return Node{n.d - 1, {n.o[0] >> 1, n.o[1] >> 1, n.o[2] >> 1}};
+}
+// ReconOctNode::getCornerIndex (ReconOctNode.cpp:95-105)
+CAGR_HD inline int64_t corner_key(const View& t, const Node& n, int c) {
+    int64_t off[3];
+    for (int k = 0; k < 3; ++k) off[k] = ((int64_t)n.o[k] + ((c >> k) & 1)) << (t.md - n.d + 1);
+    return off[0] | (off[1] << 15) | (off[2] << 30);
+}
+CAGR_HD inline int32_t corner_slot(const View& t, const Node& n, int c) {
+    const int64_t k = corner_key(t, n, c);
+    const int64_t i = lower_bound(t.ck, t.n_ck, k);
+    return (i < t.n_ck && t.ck[i] == k) ? t.ck_idx[i] : -1;
+}
+// MarchingCubes::GetIndex (MarchingCubes.cpp:472-483) at the node's corners
+CAGR_HD inline int mc_index(const View& t, const Node& n) {
+    int idx = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int32_t s = corner_slot(t, n, c);
+        if (s >= 0 && t.values[s] < t.iso) idx |= 1 << corner_bit(c);
+    }
+    return idx;
+}
+
+// same-depth neighbour across face f / edge e, when it is an internal node
+CAGR_HD inline bool face_nbr_internal(const View& t, const Node& n, int f, Node& nb) {
+    nb = n;
+    nb.o[f >> 1] += (f & 1) ? 1 : -1;
+    return is_internal(t, nb);
+}
+CAGR_HD inline bool edge_nbr_internal(const View& t, const Node& n, int e, Node& nb) {
+    const int o = e >> 2, i = e & 1, j = (e >> 1) & 1;
+    const int a = (o == 0) ? 1 : 0, b = (o == 2) ? 1 : 2;
+    nb = n;
+    nb.o[a] += i ? 1 : -1;
+    nb.o[b] += j ? 1 : -1;
+    return is_internal(t, nb);
+}
+
+// Octree::GetRootIndex (Octree.cpp:614-685): the finest edge segment holding
+// the root of edge e of node n.  The reference recursion (neighbour with
+// children first, then child c1 before c2) runs here on an explicit stack.
+CAGR_HD inline bool root_index(const View& t, const Node& n0, int e0, RootInfo& ri) {
+    Node sn[kMaxWalk];
+    int se[kMaxWalk];
+    int top = 0;
+    sn[top] = n0;
+    se[top++] = e0;
+    while (top > 0) {
+        --top;
+        const Node n = sn[top];
+        const int e = se[top];
+        if (!has_edge_roots(mc_index(t, n), e)) continue;
+        int f1, f2;
+        faces_of_edge(e, f1, f2);
+        Node fin = n, nb;
+        int fe = e;
+        bool finer = false;
+        if (n.d < t.md) {
+            if (face_nbr_internal(t, n, f1, nb)) { fin = nb; fe = face_reflect_edge(e, f1); finer = true; }
+            else if (face_nbr_internal(t, n, f2, nb)) { fin = nb; fe = face_reflect_edge(e, f2); finer = true; }
+            else if (edge_nbr_internal(t, n, e, nb)) { fin = nb; fe = edge_reflect_edge(e); finer = true; }
+        }
+        if (!finer) finer = is_internal(t, n);
+        int c1, c2;
+        edge_corners(fe, c1, c2);
+        if (finer) {
+            if (top + 2 > kMaxWalk) return false;
+            sn[top] = child(fin, c2); se[top++] = fe;
+            sn[top] = child(fin, c1); se[top++] = fe;
+            continue;
+        }
+        const int o = fe >> 2, i1 = fe & 1, i2 = (fe >> 1) & 1;
+        const int a = (o == 0) ? 1 : 0, b = (o == 2) ? 1 : 2;
+        const int sh = t.md - fin.d + 1;
+        const int64_t offset = ((int64_t)1 << fin.d) + fin.o[o] - 1;
+        const int64_t ea = (int64_t)(fin.o[a] + i1) << sh, eb = (int64_t)(fin.o[b] + i2) << sh;
+        ri.node = fin;
+        ri.edge = fe;
+        ri.key = (int64_t)o | (ea << 5) | (eb << 25) | (offset << 45);
+        return true;
+    }
+    return false;
+}
+
+// Octree::GetRoot (Octree.cpp:750-804): the root on the edge, interpolated
+// with the smoothing widths, in [0,1]^3 coordinates
+CAGR_HD inline void root_position(const View& t, const RootInfo& ri, float p[3]) {
+    int c1, c2;
+    edge_corners(ri.edge, c1, c2);
+    const int32_t s1 = corner_slot(t, ri.node, c1), s2 = corner_slot(t, ri.node, c2);
+    float value1 = t.values[s1], value2 = t.values[s2];
+    const float width1 = t.smooth_width, width2 = t.smooth_width;
+    const int64_t k1 = corner_key(t, ri.node, c1), k2 = corner_key(t, ri.node, c2);
+    const float a[3] = {(float)(k1 & 0xFFFF) * t.grid, (float)((k1 >> 15) & 0xFFFF) * t.grid,
+                        (float)(k1 >> 30) * t.grid};
+    const float b[3] = {(float)(k2 & 0xFFFF) * t.grid, (float)((k2 >> 15) & 0xFFFF) * t.grid,
+                        (float)(k2 >> 30) * t.grid};
+    const float isov = -t.iso;
+    value1 *= -1;
+    value2 *= -1;
+    float ratio = (value1 - isov) * width1 / ((value1 - isov) * width1 - (value2 - isov) * width2);
+    if (ratio > 1.0) ratio = 1.0;
+    if (ratio < 0.0) ratio = 0.0;
+    for (int k = 0; k < 3; ++k) p[k] = a[k] - ratio * (a[k] - b[k]);
+}
+
+// the vertex candidates of a leaf in the reference's order
+// (SetMCRootPositions, Octree.cpp:814-850: orientation, then i, then j)
+CAGR_HD inline int leaf_roots(const View& t, const Node& leaf, int64_t keys[12], RootInfo* infos) {
+    if (!has_roots(mc_index(t, leaf))) return 0;
+    int n = 0;
+    for (int o = 0; o < 3; ++o)
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+                RootInfo ri;
+                const int e = i | (j << 1) | (o << 2);
+                keys[n] = root_index(t, leaf, e, ri) ? ri.key : -1;
+                if (infos) infos[n] = ri;
+                ++n;
+            }
+    return n;
+}
+
+// Octree::GetRootPair (Octree.cpp:881-898)
+CAGR_HD inline bool root_pair(const View& t, const RootInfo& ri, RootInfo& pair) {
+    Node n = ri.node;
+    int c1, c2;
+    edge_corners(ri.edge, c1, c2);
+    while (n.d > 0) {
+        const int c = child_index(n);
+        if (c != c1 && c != c2) return false;
+        const Node p = parent(n);
+        if (!has_edge_roots(mc_index(t, p), ri.edge))
+            return root_index(t, child(p, c == c1 ? c2 : c1), ri.edge, pair);
+        n = p;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ per-leaf polygons
+// workspace of one leaf (caller-provided; device code points it at global memory)
+struct Work {
+    int64_t ea[kMaxEdges], eb[kMaxEdges];     // iso edges (first, second) by root key
+    int ne;
+    int64_t vkey[kMaxVerts];                   // vertexCount of the reference
+    int vcount[kMaxVerts];
+    RootInfo vinfo[kMaxVerts];
+    int nv;
+    int64_t loop[kMaxEdges];                   // one loop's vertex keys
+    int64_t back[kMaxEdges];                   // its back chain while it is built
+    // triangulation (Geometry.cpp:74-220)
+    float tp[kMaxEdges][3];
+    int te_p[3 * kMaxEdges][2], te_t[3 * kMaxEdges][2];
+    int tt_e[kMaxEdges][3];
+    int nte, ntt;
+    bool overflow;
+};
+
+CAGR_HD inline int vfind(const Work& w, int64_t key) {
+    for (int i = 0; i < w.nv; ++i)
+        if (w.vkey[i] == key) return i;
+    return -1;
+}
+CAGR_HD inline int vget(Work& w, const RootInfo& ri) {
+    int i = vfind(w, ri.key);
+    if (i >= 0) return i;
+    if (w.nv >= kMaxVerts) { w.overflow = true; return -1; }
+    i = w.nv++;
+    w.vkey[i] = ri.key;
+    w.vcount[i] = 0;
+    w.vinfo[i] = ri;
+    return i;
+}
+CAGR_HD inline void push_edge(Work& w, int64_t a, int64_t b) {
+    if (w.ne >= kMaxEdges) { w.overflow = true; return; }
+    w.ea[w.ne] = a;
+    w.eb[w.ne] = b;
+    ++w.ne;
+}
+
+// iso edges of leaf n1 on face f (FaceEdgesFunction::Function, Octree.cpp:1203-1248)
+CAGR_HD inline void face_edges_of(const View& t, const Node& n1, int f, Work& w) {
+    if (is_internal(t, n1)) return;
+    const int mc = mc_index(t, n1);
+    if (!has_roots(mc)) return;
+    const int8_t* tri = kTriTable[mc];
+    for (int j = 0; j < 15 && tri[j] >= 0; j += 3)
+        for (int k = 0; k < 3; ++k) {
+            const int e1 = tri[j + k], e2 = tri[j + (k + 1) % 3];
+            if (face_of_edges(e1, e2) != f) continue;
+            RootInfo r1, r2;
+            if (root_index(t, n1, e1, r1) && root_index(t, n1, e2, r2)) {
+                push_edge(w, r2.key, r1.key);
+                const int i1 = vget(w, r1), i2 = vget(w, r2);
+                if (i1 < 0 || i2 < 0) return;
+                w.vcount[i1]--;
+                w.vcount[i2]++;
+            }
+        }
+}
+
+// Octree::GetMCIsoEdges (Octree.cpp:900-1006)
+CAGR_HD inline void iso_edges(const View& t, const Node& n, Work& w) {
+    w.ne = 0;
+    w.nv = 0;
+    const int mc = mc_index(t, n);
+    const int8_t* tri = kTriTable[mc];
+    for (int f = 0; f < 6; ++f) {
+        const int ref = f ^ 1;
+        Node nb;
+        const bool boundary = n.d <= 0 ||
+                              (((n.o[f >> 1] << 1) + ((f & 1) << 1)) % (2 << n.d)) == 0;
+        if (face_nbr_internal(t, n, f, nb) && !boundary) {
+            // ReconOctNode::processNodeFaces (ReconOctNode.cpp:303-329): the
+            // 4 children on face `ref` level by level, then each one's own
+            int fc[4];
+            face_corners(ref, fc);
+            Node st[kMaxWalk];
+            int top = 0;
+            st[top++] = nb;
+            while (top > 0) {
+                const Node p = st[--top];
+                Node ch[4];
+                for (int q = 0; q < 4; ++q) {
+                    ch[q] = child(p, fc[q]);
+                    face_edges_of(t, ch[q], ref, w);
+                }
+                for (int q = 3; q >= 0; --q)
+                    if (is_internal(t, ch[q])) {
+                        if (top >= kMaxWalk) { w.overflow = true; return; }
+                        st[top++] = ch[q];
+                    }
+            }
+        } else {
+            for (int j = 0; j < 15 && tri[j] >= 0; j += 3)
+                for (int k = 0; k < 3; ++k) {
+                    const int e1 = tri[j + k], e2 = tri[j + (k + 1) % 3];
+                    if (face_of_edges(e1, e2) != f) continue;
+                    RootInfo r1, r2;
+                    if (root_index(t, n, e1, r1) && root_index(t, n, e2, r2)) {
+                        push_edge(w, r1.key, r2.key);
+                        const int i1 = vget(w, r1), i2 = vget(w, r2);
+                        if (i1 < 0 || i2 < 0) return;
+                        w.vcount[i1]++;
+                        w.vcount[i2]--;
+                    }
+                }
+        }
+    }
+    // close the loops across finer neighbours (Octree.cpp:966-1005)
+    for (int i = 0; i < w.ne; ++i) {
+        int v = vfind(w, w.ea[i]);
+        if (v >= 0 && w.vcount[v]) {
+            RootInfo pr;
+            if (root_pair(t, w.vinfo[v], pr)) {
+                const int pv = vfind(w, pr.key);
+                if (pv >= 0) {
+                    push_edge(w, pr.key, w.ea[i]);
+                    w.vcount[pv]++;
+                    w.vcount[v]--;
+                }
+            }
+        }
+        v = vfind(w, w.eb[i]);
+        if (v >= 0 && w.vcount[v]) {
+            RootInfo pr;
+            if (root_pair(t, w.vinfo[v], pr)) {
+                const int pv = vfind(w, pr.key);
+                if (pv >= 0) {
+                    push_edge(w, w.eb[i], pr.key);
+                    w.vcount[v]++;
+                    w.vcount[pv]--;
+                }
+            }
+        }
+        if (w.overflow) return;
+    }
+}
+
+// Octree::GetEdgeLoops (Octree.cpp:1008-1062): removes the next loop from
+// the edge list (swap-with-last removal, as the reference) into w.loop;
+// returns its length, 0 when no edge is left
+CAGR_HD inline int next_loop(Work& w) {
+    if (w.ne == 0) return 0;
+    const int64_t e0a = w.ea[0], e0b = w.eb[0];
+    w.ea[0] = w.ea[w.ne - 1];
+    w.eb[0] = w.eb[w.ne - 1];
+    --w.ne;
+    int64_t front_idx = e0b, back_idx = e0a;
+    // front / back chains kept in the loop buffer from both ends
+    int nf = 0, nb = 0;
+    int64_t* fr = w.loop;                          // front[0..nf) firsts
+    int64_t* back = w.back;
+    for (int j = w.ne - 1; j >= 0; --j) {
+        if (w.ea[j] == front_idx || w.eb[j] == front_idx) {
+            int64_t first, second;
+            if (w.ea[j] == front_idx) { first = w.ea[j]; second = w.eb[j]; }
+            else { first = w.eb[j]; second = w.ea[j]; }
+            front_idx = second;
+            fr[nf++] = first;
+            w.ea[j] = w.ea[w.ne - 1];
+            w.eb[j] = w.eb[w.ne - 1];
+            --w.ne;
+            j = w.ne;
+        } else if (w.ea[j] == back_idx || w.eb[j] == back_idx) {
+            int64_t first, second;
+            if (w.eb[j] == back_idx) { first = w.ea[j]; second = w.eb[j]; }
+            else { first = w.eb[j]; second = w.ea[j]; }
+            (void)second;
+            back_idx = first;
+            back[nb++] = first;
+            w.ea[j] = w.ea[w.ne - 1];
+            w.eb[j] = w.eb[w.ne - 1];
+            --w.ne;
+            j = w.ne;
+        }
+    }
+    // loop = reversed back, e, front (firsts of each edge)
+    const int len = nb + 1 + nf;
+    for (int q = nf - 1; q >= 0; --q) w.loop[nb + 1 + q] = fr[q];
+    for (int q = 0; q < nb; ++q) w.loop[q] = back[nb - 1 - q];
+    w.loop[nb] = e0a;
+    return len;
+}
+
+// ------------------------------------------------------------ triangulation
+CAGR_HD inline double tri_area(const Work& w, int p1, int p2, int p3) {   // Triangulation::area
+    float q1[3], q2[3];
+    for (int k = 0; k < 3; ++k) {
+        q1[k] = w.tp[p2][k] - w.tp[p1][k];
+        q2[k] = w.tp[p3][k] - w.tp[p1][k];
+    }
+    const float x = q1[1] * q2[2] - q1[2] * q2[1];
+    const float y = -q1[0] * q2[2] + q1[2] * q2[0];
+    const float z = q1[0] * q2[1] - q1[1] * q2[0];
+    const double s = (double)x * (double)x + (double)y * (double)y + (double)z * (double)z;
+    return (double)(float)sqrt(s);
+}
+CAGR_HD inline int te_find(const Work& w, int p1, int p2) {
+    for (int i = 0; i < w.nte; ++i)
+        if ((w.te_p[i][0] == p1 && w.te_p[i][1] == p2) || (w.te_p[i][0] == p2 && w.te_p[i][1] == p1))
+            return i;
+    return -1;
+}
+CAGR_HD inline bool tri_factor(const Work& w, int t, int& p1, int& p2, int& p3) {
+    int p[3];
+    for (int k = 0; k < 3; ++k) {
+        const int e = w.tt_e[t][k];
+        if (e < 0) return false;
+        p[k] = (w.te_t[e][0] == t) ? w.te_p[e][0] : w.te_p[e][1];
+    }
+    p1 = p[0];
+    p2 = p[1];
+    p3 = p[2];
+    return true;
+}
+CAGR_HD inline void tri_add(Work& w, int p1, int p2, int p3) {           // Triangulation::addTriangle
+    const int p[3] = {p1, p2, p3};
+    const int t = w.ntt++;
+    w.tt_e[t][0] = w.tt_e[t][1] = w.tt_e[t][2] = -1;
+    for (int i = 0; i < 3; ++i) {
+        const int a = p[i], b = p[(i + 1) % 3];
+        int e = te_find(w, a, b);
+        if (e < 0) {
+            e = w.nte++;
+            w.te_p[e][0] = a;
+            w.te_p[e][1] = b;
+            w.te_t[e][0] = t;
+            w.te_t[e][1] = -1;
+        } else if (w.te_p[e][0] == a) {
+            if (w.te_t[e][0] < 0) w.te_t[e][0] = t;
+            else return;
+        } else {
+            if (w.te_t[e][1] < 0) w.te_t[e][1] = t;
+            else return;
+        }
+        w.tt_e[t][i] = e;
+    }
+}
+CAGR_HD inline bool tri_flip(Work& w, int e) {                           // Triangulation::flipMinimize
+    if (w.te_t[e][0] < 0 || w.te_t[e][1] < 0) return false;
+    int P[3], Q[3];
+    if (!tri_factor(w, w.te_t[e][0], P[0], P[1], P[2])) return false;
+    if (!tri_factor(w, w.te_t[e][1], Q[0], Q[1], Q[2])) return false;
+    const double old_area = tri_area(w, P[0], P[1], P[2]) + tri_area(w, Q[0], Q[1], Q[2]);
+    int ip, iq;
+    for (ip = 0; ip < 3; ++ip) {
+        int i;
+        for (i = 0; i < 3; ++i) if (P[ip] == Q[i]) break;
+        if (i == 3) break;
+    }
+    for (iq = 0; iq < 3; ++iq) {
+        int i;
+        for (i = 0; i < 3; ++i) if (P[i] == Q[iq]) break;
+        if (i == 3) break;
+    }
+    if (ip == 3 || iq == 3) return false;
+    const int nP[3] = {P[ip], P[(ip + 1) % 3], Q[iq]};
+    const int nQ[3] = {Q[iq], P[(ip + 2) % 3], P[ip]};
+    const double new_area = tri_area(w, nP[0], nP[1], nP[2]) + tri_area(w, nQ[0], nQ[1], nQ[2]);
+    if (old_area <= new_area) return false;
+    w.te_p[e][0] = nP[0];
+    w.te_p[e][1] = nQ[0];
+    const int t0 = w.te_t[e][0], t1 = w.te_t[e][1];
+    for (int i = 0; i < 3; ++i) {
+        int idx = te_find(w, nQ[i], nQ[(i + 1) % 3]);
+        w.tt_e[t0][i] = idx;
+        if (idx != e) {
+            if (w.te_t[idx][0] == t1) w.te_t[idx][0] = t0;
+            if (w.te_t[idx][1] == t1) w.te_t[idx][1] = t0;
+        }
+        idx = te_find(w, nP[i], nP[(i + 1) % 3]);
+        w.tt_e[t1][i] = idx;
+        if (idx != e) {
+            if (w.te_t[idx][0] == t0) w.te_t[idx][0] = t1;
+            if (w.te_t[idx][1] == t0) w.te_t[idx][1] = t1;
+        }
+    }
+    return true;
+}
+
+}  // namespace mc
+}  // namespace cagr
+
+namespace cagr {
+namespace mc {
+
+// the triangles of one leaf (Octree::GetMCIsoTriangles(node, ...) with
+// AddTriangles, Octree.cpp:1064-1200): each loop of iso edges becomes a fan
+// from its first vertex, flipped to minimum area; `vindex` maps a root key to
+// its vertex number (-1 if none: the reference skips such a vertex).  Emits
+// (a, b, c) through `emit` in the reference's order; returns the count.
+template <class VIndex, class Emit>
+CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VIndex vindex,
+                                  Emit emit) {
+    w.overflow = false;
+    iso_edges(t, leaf, w);
+    if (w.overflow) return -1;
+    int count = 0;
+    int len;
+    while ((len = next_loop(w)) > 0) {
+        int vid[kMaxEdges];
+        int nv = 0;
+        for (int j = 0; j < len; ++j) {
+            const int v = vindex(w.loop[j]);
+            if (v >= 0) vid[nv++] = v;
+        }
+        if (nv > 3) {
+            w.nte = w.ntt = 0;
+            for (int j = 0; j < nv; ++j) vindex.position(vid[j], w.tp[j]);
+            for (int j = 1; j < nv - 1; ++j) tri_add(w, 0, j, j + 1);
+            for (;;) {
+                int e;
+                for (e = 0; e < w.nte; ++e)
+                    if (tri_flip(w, e)) break;
+                if (e == w.nte) break;
+            }
+            for (int q = 0; q < w.ntt; ++q) {
+                int a = 0, b = 0, c = 0;
+                tri_factor(w, q, a, b, c);
+                emit(vid[a], vid[b], vid[c]);
+                ++count;
+            }
+        } else if (nv == 3) {
+            emit(vid[0], vid[1], vid[2]);
+            ++count;
+        }
+    }
+    return count;
+}
+
+}  // namespace mc
+}  // namespace cagr
