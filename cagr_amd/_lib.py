@@ -34,7 +34,7 @@ EXPORTS = (
     "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
     "cagr_order_items", "cagr_profile", "cagr_profile_read", "cagr_adjoint_plan",
     "cagr_apply_adjoint_planned", "cagr_pgr_build", "cagr_pgr_destroy", "cagr_pgr_info",
-    "cagr_pgr_read",
+    "cagr_pgr_read", "cagr_pgr_box", "cagr_pgr_mesh", "cagr_pgr_mesh_read",
 )
 
 _P, _I, _D, _C = ct.c_void_p, ct.c_int64, ct.c_double, ct.c_int
@@ -65,6 +65,9 @@ _SIGS = {
     "cagr_pgr_destroy": (None, [_P]),
     "cagr_pgr_info": (_C, [_P, _P, _P, _P, _P]),
     "cagr_pgr_read": (_C, [_P, _P, _P, _P, _P]),
+    "cagr_pgr_box": (_C, [_P, _P]),
+    "cagr_pgr_mesh": (_C, [_P, _P, ct.c_float, ct.c_float, _P, _P, _P]),
+    "cagr_pgr_mesh_read": (_C, [_P, _P, _P, _P]),
 }
 
 _lib = None

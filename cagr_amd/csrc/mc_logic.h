@@ -26,6 +26,11 @@
 
 #include "mc_tables.h"
 
+// every float expression rounds each operation, as the reference's x86 build
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
 #ifndef CAGR_HD
 #define CAGR_HD __host__ __device__
 #endif
@@ -34,9 +39,10 @@ namespace cagr {
 namespace mc {
 
 // limits of the per-leaf work (checked; exceeding them is reported)
-constexpr int kMaxEdges = 512;     // iso edges of one leaf incl. finer neighbours
-constexpr int kMaxVerts = 512;     // distinct roots met by one leaf
-constexpr int kMaxWalk = 64;       // explicit stacks of the descents
+// (test clouds peak at 19 edges and 19 roots per leaf)
+constexpr int kMaxEdges = 128;     // iso edges of one leaf incl. finer neighbours
+constexpr int kMaxVerts = 128;     // distinct roots met by one leaf
+constexpr int kMaxWalk = 48;       // explicit stacks of the descents (depth <= 13)
 
 struct Node {
     int d;
@@ -299,6 +305,7 @@ struct Work {
     int nv;
     int64_t loop[kMaxEdges];                   // one loop's vertex keys
     int64_t back[kMaxEdges];                   // its back chain while it is built
+    int vid[kMaxEdges];                        // its vertex numbers
     // triangulation (Geometry.cpp:74-220)
     float tp[kMaxEdges][3];
     int te_p[3 * kMaxEdges][2], te_t[3 * kMaxEdges][2];
@@ -581,7 +588,7 @@ namespace mc {
 // from its first vertex, flipped to minimum area; `vindex` maps a root key to
 // its vertex number (-1 if none: the reference skips such a vertex).  Emits
 // (a, b, c) through `emit` in the reference's order; returns the count.
-template <class VIndex, class Emit>
+template <bool kCountOnly, class VIndex, class Emit>
 CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VIndex vindex,
                                   Emit emit) {
     w.overflow = false;
@@ -590,11 +597,15 @@ CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VInd
     int count = 0;
     int len;
     while ((len = next_loop(w)) > 0) {
-        int vid[kMaxEdges];
+        int* vid = w.vid;
         int nv = 0;
         for (int j = 0; j < len; ++j) {
             const int v = vindex(w.loop[j]);
             if (v >= 0) vid[nv++] = v;
+        }
+        if (kCountOnly) {
+            if (nv >= 3) count += nv - 2;    // a fan of nv-2 triangles, flips keep the count
+            continue;
         }
         if (nv > 3) {
             w.nte = w.ntt = 0;

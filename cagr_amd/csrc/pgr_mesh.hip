@@ -1,0 +1,308 @@
+// Marching cubes on the PGR octree (GaussRecon's Octree::GetMCIsoTriangles
+// with fullDepthIso 0, nonLinearFit 1, triangles out; gaussrecon_src/
+// Octree.cpp:518-1257), on the GPU, one leaf per thread, with the per-leaf
+// logic of mc_logic.h.
+//
+// Vertices: every leaf with roots lists its 12 edge roots (key of the finest
+// edge segment, in the reference's slot order); the valid keys are sorted
+// stably, the first occurrence of each key over (leaf preorder, slot) gets
+// its vertex number by a scan -- the order the reference's roots map fills
+// in -- and that occurrence computes the position.
+// Triangles: a count pass and a write pass over the leaves (persistent
+// threads, one workspace each in global memory) give every leaf its loops,
+// the loops their fans and area-minimising flips, at offsets from a scan,
+// so the triangle list comes out in the reference's order.
+#include <algorithm>
+
+#include "common.h"
+#include "mc_logic.h"
+#include "pgr.h"
+
+#pragma clang fp contract(off)
+
+namespace cagr {
+namespace {
+
+using namespace mc;
+
+constexpr int kT = 256;
+inline int64_t nblk(int64_t n) { return (n + kT - 1) / kT; }
+
+__device__ inline Node leaf_node(const int32_t* lv, int64_t r) {
+    return Node{lv[4 * r], {lv[4 * r + 1], lv[4 * r + 2], lv[4 * r + 3]}};
+}
+
+__global__ __launch_bounds__(kT) void k_mc_roots(View t, const int32_t* __restrict__ leaves,
+                                                 int64_t nl, uint64_t* __restrict__ keys,
+                                                 int32_t* __restrict__ valid) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r > nl) return;
+    if (r == nl) {
+        valid[12 * nl] = 0;
+        return;
+    }
+    int64_t rk[12];
+    const int cnt = leaf_roots(t, leaf_node(leaves, r), rk, nullptr);
+    for (int s = 0; s < 12; ++s) {
+        const bool ok = s < cnt && rk[s] >= 0;
+        keys[12 * r + s] = ok ? (uint64_t)rk[s] : ~0ull;
+        valid[12 * r + s] = ok ? 1 : 0;
+    }
+}
+
+// the valid root records (key, occurrence), in occurrence order
+__global__ __launch_bounds__(kT) void k_mc_compact(const uint64_t* __restrict__ keys,
+                                                   const int32_t* __restrict__ pos, int64_t n12,
+                                                   uint64_t* __restrict__ ckeys,
+                                                   int32_t* __restrict__ cocc) {
+    const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (q >= n12 || pos[q + 1] == pos[q]) return;
+    ckeys[pos[q]] = keys[q];
+    cocc[pos[q]] = (int32_t)q;
+}
+
+// heads of the sorted valid keys: first[occ] = 1 at each key's first occurrence
+__global__ __launch_bounds__(kT) void k_mc_first(const uint64_t* __restrict__ keys,
+                                                 const int32_t* __restrict__ occ, int64_t m,
+                                                 int32_t* __restrict__ first,
+                                                 int32_t* __restrict__ head) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r > m) return;
+    if (r == m) {
+        head[m] = 0;
+        return;
+    }
+    const bool valid = keys[r] != ~0ull;
+    const bool h = valid && (r == 0 || keys[r - 1] != keys[r]);
+    head[r] = h ? 1 : 0;
+    if (h) first[occ[r]] = 1;
+}
+
+__global__ __launch_bounds__(kT) void k_fill(int32_t* __restrict__ a, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r < n) a[r] = 0;
+}
+
+// vertex keys in key order with their vertex numbers
+__global__ __launch_bounds__(kT) void k_mc_vkeys(const uint64_t* __restrict__ keys,
+                                                 const int32_t* __restrict__ occ, int64_t m,
+                                                 const int32_t* __restrict__ hrank,
+                                                 const int32_t* __restrict__ pos,
+                                                 int64_t* __restrict__ vkeys,
+                                                 int32_t* __restrict__ vidx) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= m || hrank[r + 1] == hrank[r]) return;
+    vkeys[hrank[r]] = (int64_t)keys[r];
+    vidx[hrank[r]] = pos[occ[r]];
+}
+
+__global__ __launch_bounds__(kT) void k_mc_vertices(View t, const int32_t* __restrict__ leaves,
+                                                    const int32_t* __restrict__ pos, int64_t n12,
+                                                    float* __restrict__ verts) {
+    const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (q >= n12 || pos[q + 1] == pos[q]) return;
+    const int s = (int)(q % 12);
+    const int o = s >> 2, i = (s >> 1) & 1, j = s & 1;
+    RootInfo ri;
+    if (!root_index(t, leaf_node(leaves, q / 12), i | (j << 1) | (o << 2), ri)) return;
+    float p[3];
+    root_position(t, ri, p);
+    float* v = verts + 3 * (int64_t)pos[q];
+    v[0] = p[0];
+    v[1] = p[1];
+    v[2] = p[2];
+}
+
+struct DevVIndex {
+    const int64_t* vkeys;
+    const int32_t* vidx;
+    int64_t nvk;
+    const float* verts;
+    __device__ int operator()(int64_t k) const {
+        const int64_t i = lower_bound(vkeys, nvk, k);
+        return (i < nvk && vkeys[i] == k) ? vidx[i] : -1;
+    }
+    __device__ void position(int v, float p[3]) const {
+        for (int k = 0; k < 3; ++k) p[k] = verts[3 * (int64_t)v + k];
+    }
+};
+
+// count (tris == nullptr) or write the triangles of every leaf; one
+// workspace per thread, leaves by grid stride
+template <bool kCountOnly>
+__global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restrict__ leaves,
+                                                int64_t nl, DevVIndex vi, Work* __restrict__ ws,
+                                                int32_t* __restrict__ counts,
+                                                const int64_t* __restrict__ offs,
+                                                int32_t* __restrict__ tris,
+                                                int32_t* __restrict__ err) {
+    const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t nthreads = (int64_t)gridDim.x * 64;
+    Work& w = ws[tid];
+    for (int64_t r = tid; r < nl; r += nthreads) {
+        int32_t* out = kCountOnly ? nullptr : tris + 3 * offs[r];
+        int k = 0;
+        const int c = leaf_triangles<kCountOnly>(t, leaf_node(leaves, r), w, vi,
+                                                 [&](int a, int b, int cc) {
+                                                     out[3 * k] = a;
+                                                     out[3 * k + 1] = b;
+                                                     out[3 * k + 2] = cc;
+                                                     ++k;
+                                                 });
+        if (c < 0) {
+            atomicOr(err, 1);
+            if (kCountOnly) counts[r] = 0;
+            continue;
+        }
+        if (kCountOnly) counts[r] = c;
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_widen(const int32_t* __restrict__ c, int64_t n,
+                                              int64_t* __restrict__ o) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r < n) o[r] = c[r];
+}
+
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        CAGR_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        return CAGR_OK;
+    }
+};
+
+int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t st) {
+    int rc;
+    View t{};
+    t.md = g->depth;
+    t.in_keys = g->in_keys;
+    t.n_in = g->n_in;
+    t.ck = g->ck;
+    t.ck_idx = g->ck_idx;
+    t.n_ck = g->n_corners;
+    t.values = values;
+    t.iso = iso;
+    t.smooth_width = sw;
+    t.grid = (float)(1.0f / std::pow(2.0, t.md + 1));
+    const int64_t nl = g->n_leaves, n12 = 12 * nl;
+    CAGR_REQUIRE(n12 < (int64_t(1) << 31), CAGR_ERR_ARG, "too many leaves for the mesh");
+
+    // vertices: root keys per (leaf, slot), the valid ones compacted and
+    // sorted stably by key
+    Buf<uint64_t> all, keys, ktmp;
+    Buf<int32_t> valid, occ, otmp, first, head;
+    Buf<uint32_t> hist;
+    Buf<int32_t> stmp;
+    if ((rc = all.alloc(n12)) || (rc = valid.alloc(n12 + 1)) || (rc = first.alloc(n12 + 1)) ||
+        (rc = stmp.alloc(scan_tmp_len(n12 + 1))))
+        return rc;
+    k_mc_roots<<<nblk(nl + 1), kT, 0, st>>>(t, g->leaves, nl, all.p, valid.p);
+    CAGR_LAUNCHED();
+    if ((rc = scan_exclusive<int32_t>(valid.p, n12 + 1, stmp.p, st))) return rc;
+    int32_t m32 = 0;
+    CAGR_HIP(hipMemcpyAsync(&m32, valid.p + n12, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    const int64_t m = m32;
+    if ((rc = keys.alloc(m)) || (rc = ktmp.alloc(m)) || (rc = occ.alloc(m)) ||
+        (rc = otmp.alloc(m)) || (rc = head.alloc(m + 1)) || (rc = hist.alloc(sort_hist_len(m))))
+        return rc;
+    k_mc_compact<<<nblk(n12), kT, 0, st>>>(all.p, valid.p, n12, keys.p, occ.p);
+    CAGR_LAUNCHED();
+    if ((rc = sort_pairs_u64(keys.p, occ.p, ktmp.p, otmp.p, hist.p, m, 64, st))) return rc;
+    k_fill<<<nblk(n12 + 1), kT, 0, st>>>(first.p, n12 + 1);
+    CAGR_LAUNCHED();
+    k_mc_first<<<nblk(m + 1), kT, 0, st>>>(keys.p, occ.p, m, first.p, head.p);
+    CAGR_LAUNCHED();
+    if ((rc = scan_exclusive<int32_t>(first.p, n12 + 1, stmp.p, st))) return rc;
+    if ((rc = scan_exclusive<int32_t>(head.p, m + 1, stmp.p, st))) return rc;
+    int32_t nv = 0;
+    CAGR_HIP(hipMemcpyAsync(&nv, first.p + n12, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    if (g->mesh_v) CAGR_HIP(hipFree(g->mesh_v));
+    if (g->mesh_t) CAGR_HIP(hipFree(g->mesh_t));
+    g->mesh_v = nullptr;
+    g->mesh_t = nullptr;
+    g->n_vertices = g->n_triangles = 0;
+    CAGR_HIP(hipMalloc(&g->mesh_v, 3 * std::max<int64_t>(nv, 1) * sizeof(float)));
+    k_mc_vertices<<<nblk(n12), kT, 0, st>>>(t, g->leaves, first.p, n12, g->mesh_v);
+    CAGR_LAUNCHED();
+    Buf<int64_t> vkeys;
+    Buf<int32_t> vidx;
+    if ((rc = vkeys.alloc(nv)) || (rc = vidx.alloc(nv))) return rc;
+    k_mc_vkeys<<<nblk(m), kT, 0, st>>>(keys.p, occ.p, m, head.p, first.p, vkeys.p, vidx.p);
+    CAGR_LAUNCHED();
+
+    // triangles: count, offsets, write
+    const int64_t nthreads = std::min<int64_t>(nl, 256 * 64 * 2);
+    const int64_t nb = (nthreads + 63) / 64;
+    Buf<Work> ws;
+    Buf<int32_t> cnt, err;
+    Buf<int64_t> offs;
+    if ((rc = ws.alloc(nb * 64)) || (rc = cnt.alloc(nl + 1)) || (rc = err.alloc(1)) ||
+        (rc = offs.alloc(nl + 1)))
+        return rc;
+    k_fill<<<1, kT, 0, st>>>(err.p, 1);
+    CAGR_LAUNCHED();
+    const DevVIndex vi{vkeys.p, vidx.p, nv, g->mesh_v};
+    k_mc_tris<true><<<nb, 64, 0, st>>>(t, g->leaves, nl, vi, ws.p, cnt.p, nullptr, nullptr, err.p);
+    CAGR_LAUNCHED();
+    k_fill<<<1, kT, 0, st>>>(cnt.p + nl, 1);
+    CAGR_LAUNCHED();
+    k_widen<<<nblk(nl + 1), kT, 0, st>>>(cnt.p, nl + 1, offs.p);
+    CAGR_LAUNCHED();
+    Buf<int64_t> otmp64;
+    if ((rc = otmp64.alloc(scan_tmp_len(nl + 1)))) return rc;
+    if ((rc = scan_exclusive<int64_t>(offs.p, nl + 1, otmp64.p, st))) return rc;
+    int64_t nt = 0;
+    int32_t e = 0;
+    CAGR_HIP(hipMemcpyAsync(&nt, offs.p + nl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipMemcpyAsync(&e, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    CAGR_REQUIRE(e == 0, CAGR_ERR_STATE,
+                 "a leaf exceeded the marching-cubes workspace (128 iso edges or roots)");
+    CAGR_HIP(hipMalloc(&g->mesh_t, 3 * std::max<int64_t>(nt, 1) * sizeof(int32_t)));
+    k_mc_tris<false><<<nb, 64, 0, st>>>(t, g->leaves, nl, vi, ws.p, nullptr, offs.p, g->mesh_t,
+                                         err.p);
+    CAGR_LAUNCHED();
+    CAGR_HIP(hipStreamSynchronize(st));
+    g->n_vertices = nv;
+    g->n_triangles = nt;
+    return CAGR_OK;
+}
+
+}  // namespace
+}  // namespace cagr
+
+using namespace cagr;
+
+extern "C" {
+
+int cagr_pgr_mesh(cagr_pgr* g, const float* values, float iso, float smooth_width, void* stream,
+                  int64_t* n_vertices, int64_t* n_triangles) {
+    CAGR_REQUIRE(g != nullptr, CAGR_ERR_ARG, "null PGR octree handle");
+    CAGR_REQUIRE(values != nullptr, CAGR_ERR_ARG, "null corner values");
+    const int rc = pgr_mesh(g, values, iso, smooth_width, (hipStream_t)stream);
+    if (rc) return rc;
+    if (n_vertices) *n_vertices = g->n_vertices;
+    if (n_triangles) *n_triangles = g->n_triangles;
+    return CAGR_OK;
+}
+
+int cagr_pgr_mesh_read(const cagr_pgr* g, float* vertices, int32_t* triangles, void* stream) {
+    CAGR_REQUIRE(g != nullptr, CAGR_ERR_ARG, "null PGR octree handle");
+    const hipStream_t st = (hipStream_t)stream;
+    if (vertices && g->n_vertices)
+        CAGR_HIP(hipMemcpyAsync(vertices, g->mesh_v, 3 * g->n_vertices * sizeof(float),
+                                hipMemcpyDeviceToDevice, st));
+    if (triangles && g->n_triangles)
+        CAGR_HIP(hipMemcpyAsync(triangles, g->mesh_t, 3 * g->n_triangles * sizeof(int32_t),
+                                hipMemcpyDeviceToDevice, st));
+    return CAGR_OK;
+}
+
+}  // extern "C"

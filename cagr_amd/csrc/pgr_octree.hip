@@ -40,18 +40,7 @@
 
 #pragma clang fp contract(off)
 
-struct cagr_pgr {
-    int device = 0;
-    int64_t n = 0;
-    int64_t n_leaves = 0;
-    int64_t n_corners = 0;
-    int depth = 0;            // Octree::maxDepth after the build (tree height + 1)
-    float scale = 0.f;        // Octree::maxScale
-    float center[3] = {0.f, 0.f, 0.f};
-    float* samples = nullptr; // [n,3] in [0,1]^3
-    float* corners = nullptr; // [n_corners,3] in [0,1]^3, gridDataVector order
-    int32_t* leaves = nullptr;// [n_leaves,4] depth, x, y, z in preorder
-};
+#include "pgr.h"
 
 namespace cagr {
 namespace {
@@ -536,6 +525,20 @@ __global__ __launch_bounds__(kT) void k_corner_first(const uint64_t* __restrict_
     if (r < R && (r == 0 || keys[r] != keys[r - 1])) first[vals[r]] = 1;
 }
 
+// the corner keys in key order: ck[rank] = key, ck_idx[rank] = its
+// gridDataVector position (the run's first record is its first occurrence)
+__global__ __launch_bounds__(kT) void k_corner_index(const uint64_t* __restrict__ keys,
+                                                     const int32_t* __restrict__ vals, int64_t R,
+                                                     const int32_t* __restrict__ pos,
+                                                     const int32_t* __restrict__ rank,
+                                                     int64_t* __restrict__ ck,
+                                                     int32_t* __restrict__ ck_idx) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= R || rank[r + 1] == rank[r]) return;
+    ck[rank[r]] = (int64_t)keys[r];
+    ck_idx[rank[r]] = pos[vals[r]];
+}
+
 __global__ __launch_bounds__(kT) void k_corner_write(const int32_t* __restrict__ leaves,
                                                      const int32_t* __restrict__ pos, int64_t R,
                                                      int md, float grid,
@@ -657,6 +660,7 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
         const float ext[3] = {bb[3] - bb[0], bb[4] - bb[1], bb[5] - bb[2]};
         g->scale = std::max(std::max(ext[0], ext[1]), ext[2]) * 1.1f;
         for (int k = 0; k < 3; ++k) g->center[k] = (bb[3 + k] + bb[k] - g->scale) / 2;
+        for (int k = 0; k < 6; ++k) g->bb[k] = bb[k];
         CAGR_REQUIRE(g->scale > 0.f && std::isfinite(g->scale), CAGR_ERR_ARG,
                      "the samples must span a finite, non-empty box");
         k_normalize<<<nblk(3 * n), kT, 0, st>>>(pts, n, g->center[0], g->center[1], g->center[2],
@@ -846,7 +850,33 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
     const float grid = (float)(1.0f / std::pow(2.0, md + 1));
     k_corner_write<<<nblk(R), kT, 0, st>>>(leaves.p, w.flags.p, R, md, grid, g->corners);
     CAGR_LAUNCHED();
-    CAGR_HIP(hipStreamSynchronize(st));
+
+    // kept for the mesh: the corner keys in key order with their
+    // gridDataVector positions, and the internal nodes (depth-major, sorted)
+    {
+        DBuf<int32_t> rank;
+        if ((rc = rank.ensure(R + 1)) || (rc = w.stmp.ensure(scan_tmp_len(R + 1)))) return rc;
+        k_heads<<<nblk(R + 1), kT, 0, st>>>(w.keys.p, R, ~0ull, rank.p);
+        CAGR_LAUNCHED();
+        if ((rc = scan_exclusive<int32_t>(rank.p, R + 1, w.stmp.p, st))) return rc;
+        CAGR_HIP(hipMalloc(&g->ck, std::max<int64_t>(nc, 1) * sizeof(int64_t)));
+        CAGR_HIP(hipMalloc(&g->ck_idx, std::max<int64_t>(nc, 1) * sizeof(int32_t)));
+        k_corner_index<<<nblk(R), kT, 0, st>>>(w.keys.p, w.vals.p, R, w.flags.p, rank.p, g->ck,
+                                                g->ck_idx);
+        CAGR_LAUNCHED();
+        int64_t n_in = 0;
+        for (int d = 0; d < H; ++d) n_in += nI[d];
+        CAGR_HIP(hipMalloc(&g->in_keys, std::max<int64_t>(n_in, 1) * sizeof(uint64_t)));
+        int64_t at = 0;
+        for (int d = 0; d < H; ++d) {
+            if (nI[d])
+                CAGR_HIP(hipMemcpyAsync(g->in_keys + at, I[d].p, nI[d] * sizeof(uint64_t),
+                                        hipMemcpyDeviceToDevice, st));
+            at += nI[d];
+        }
+        g->n_in = n_in;
+        CAGR_HIP(hipStreamSynchronize(st));
+    }
 
     g->leaves = leaves.p;
     leaves.p = nullptr;
@@ -871,6 +901,11 @@ void cagr_pgr_destroy(cagr_pgr* g) {
     if (g->samples) (void)hipFree(g->samples);
     if (g->corners) (void)hipFree(g->corners);
     if (g->leaves) (void)hipFree(g->leaves);
+    if (g->in_keys) (void)hipFree(g->in_keys);
+    if (g->ck) (void)hipFree(g->ck);
+    if (g->ck_idx) (void)hipFree(g->ck_idx);
+    if (g->mesh_v) (void)hipFree(g->mesh_v);
+    if (g->mesh_t) (void)hipFree(g->mesh_t);
     if (sw) (void)hipSetDevice(cur);
     delete g;
 }
@@ -909,6 +944,12 @@ int cagr_pgr_info(const cagr_pgr* g, int64_t* n_leaves, int64_t* n_corners, int*
         scale_center[0] = g->scale;
         for (int k = 0; k < 3; ++k) scale_center[1 + k] = g->center[k];
     }
+    return CAGR_OK;
+}
+
+int cagr_pgr_box(const cagr_pgr* g, float* box) {
+    CAGR_REQUIRE(g != nullptr && box != nullptr, CAGR_ERR_ARG, "null argument");
+    for (int k = 0; k < 6; ++k) box[k] = g->bb[k];
     return CAGR_OK;
 }
 

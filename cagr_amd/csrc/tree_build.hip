@@ -396,6 +396,7 @@ int scan_exclusive(U* a, int64_t len, U* tmp, hipStream_t st) {
 }
 template int scan_exclusive<int32_t>(int32_t*, int64_t, int32_t*, hipStream_t);
 template int scan_exclusive<uint32_t>(uint32_t*, int64_t, uint32_t*, hipStream_t);
+template int scan_exclusive<int64_t>(int64_t*, int64_t, int64_t*, hipStream_t);
 
 int64_t sort_hist_len(int64_t n) {
     const int64_t h = 256 * std::max<int64_t>(1, (n + kRsTile - 1) / kRsTile);

@@ -190,3 +190,22 @@ def orient(path: str, width_config: str = "l0", *, wsmin: Optional[float] = None
         name = os.path.basename(path)[:-4]
         save_xyz(os.path.join(out_dir, name + ".xyz"), raw, out)
     return out, pn
+
+
+def save_ply_mesh(path: str, vertices, triangles) -> None:
+    """Binary little-endian PLY of a triangle mesh, byte for byte as the
+    reference's PlyWritePolygons writes it (gaussrecon_src/ply.cpp:72-143 with
+    plyfile.cpp's header): float x, y, z per vertex; per face a uchar count
+    and int indices."""
+    v = np.ascontiguousarray(vertices, dtype="<f4").reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype="<i4").reshape(-1, 3)
+    header = ("ply\nformat binary_little_endian 1.0\n"
+              f"element vertex {len(v)}\nproperty float x\nproperty float y\nproperty float z\n"
+              f"element face {len(t)}\nproperty list uchar int vertex_indices\nend_header\n")
+    faces = np.zeros(len(t), dtype=[("n", "u1"), ("i", "<i4", (3,))])
+    faces["n"] = 3
+    faces["i"] = t
+    with open(path, "wb") as f:
+        f.write(header.encode("ascii"))
+        f.write(v.tobytes())
+        f.write(faces.tobytes())

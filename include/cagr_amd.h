@@ -15,6 +15,7 @@
  *   wnt/wn_treecode_cuda/wn_treecode_cuda_torch_interface.cu:137-777 multiply_by_{A,AT,A3,G,A21,A22,A23,A2T1,A2T2,A2T3}
  *   CUDA-based/ext/gaussrecon_src/main_GaussRecon_cuda.cu:427-466     multiply_by_A3 grid query
  *   CUDA-based/ext/gaussrecon_src/Octree.cpp:77-400                   Octree::setTree (PGR query set)
+ *   CUDA-based/ext/gaussrecon_src/Octree.cpp:852-1200                 Octree::GetMCIsoTriangles (mesh)
  */
 #ifndef CAGR_AMD_H
 #define CAGR_AMD_H
@@ -228,6 +229,23 @@ int cagr_pgr_info(const cagr_pgr* pgr, int64_t* n_leaves, int64_t* n_corners, in
  * (depth, x, y, z offsets) in preorder.  Any pointer may be NULL. */
 int cagr_pgr_read(const cagr_pgr* pgr, float* samples, float* corners, int32_t* leaves,
                   void* stream);
+/* HOST: the samples' bounding box, box[6] = min xyz, max xyz (Octree::bb; the
+ * output transform of writePolygon2, Octree.cpp:1259-1267, uses it). */
+int cagr_pgr_box(const cagr_pgr* pgr, float* box);
+/* Marching cubes on the PGR octree: replaces tree.initLeaf() +
+ * tree.GetMCIsoTriangles(isoValue, &mesh, 0, 1, 0, 0)
+ * (gaussrecon_src/Octree.cpp:489-496, 852-879, 1064-1200; called at
+ * main_GaussRecon_cuda.cu:488-492).  values: [n_corners] float32, the
+ * GridData::value of every gridDataVector entry (GaussRecon sets -wn);
+ * smooth_width = GridData::smoothWidth (GaussRecon: width / 2).  The mesh
+ * stays in the handle (the previous one is replaced); synchronises the
+ * stream.  *n_vertices / *n_triangles (HOST) may be NULL. */
+int cagr_pgr_mesh(cagr_pgr* pgr, const float* values, float iso, float smooth_width, void* stream,
+                  int64_t* n_vertices, int64_t* n_triangles);
+/* device copies of the last mesh: vertices [n_vertices,3] in [0,1]^3
+ * (CoredVectorMeshData::inCorePoints), triangles [n_triangles,3] int32 (its
+ * polygons, in order).  Either pointer may be NULL. */
+int cagr_pgr_mesh_read(const cagr_pgr* pgr, float* vertices, int32_t* triangles, void* stream);
 
 #ifdef __cplusplus
 }

@@ -348,3 +348,45 @@ def pgr_ref_build(points, max_depth: int = 10, min_depth: int = 1, workdir: str 
     lv = np.zeros((nl.value, 4), np.int32)
     R.pgr_ref_leaves(_ptr(lv))
     return PGR(s, lv, g, md.value)
+
+
+def pgr_ref_mesh(values, iso: float, smooth_width: float, ply_path: str = ""):
+    """The reference's own marching cubes on the octree of the last
+    pgr_ref_build call (oracle/_ref/libpgr_ref.so): vertices [V,3] f32 in
+    [0,1]^3, polygons as (offsets [P+1], indices); writes its PLY when a path
+    is given."""
+    R = ct.CDLL(_PGR_REF_PATH)
+    R.pgr_ref_mesh.argtypes = [_P, ct.c_float, ct.c_float, ct.c_char_p, _P, _P, _P]
+    R.pgr_ref_mesh_fetch.argtypes = [_P, _P, _P]
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    nv, npo, ni = ct.c_int64(), ct.c_int64(), ct.c_int64()
+    if R.pgr_ref_mesh(_ptr(v), float(iso), float(smooth_width), ply_path.encode(), ct.byref(nv),
+                      ct.byref(npo), ct.byref(ni)):
+        raise RuntimeError("reference mesh failed (no octree built)")
+    pts = np.zeros((nv.value, 3), np.float32)
+    off = np.zeros(npo.value + 1, np.int64)
+    idx = np.zeros(ni.value, np.int32)
+    R.pgr_ref_mesh_fetch(_ptr(pts), _ptr(off), _ptr(idx))
+    return pts, off, idx
+
+
+def sphere_field(corners, centre=0.5, radius=0.3):
+    """A reproducible float32 test field: |c - centre|^2 - r^2, every step a
+    float32 operation in a fixed order."""
+    c = np.asarray(corners, np.float32)
+    d = c - np.float32(centre)
+    return ((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) - np.float32(radius * radius)
+
+
+def winding_field(samples01, normals, corners01, width=0.01, area=1e-5):
+    """GaussRecon's field from the FP64 oracle: -wn (A3, continuous) at the
+    corners and the isovalue (-wn at the samples, element N/2), as float32."""
+    p64 = (2 * np.asarray(samples01, np.float32) - 1).astype(np.float64)
+    nml = np.asarray(normals, np.float64)
+    mu = nml / np.maximum(np.linalg.norm(nml, axis=1, keepdims=True), 1e-12) * area
+    t = build_tree(p64, 15, 1)
+    q = (2 * np.asarray(corners01, np.float32) - 1).astype(np.float64)
+    vals = -apply("A3", t, p64, mu, width, queries=q, continuous=True)[:, 0]
+    ws = apply("A3", t, p64, mu, width, continuous=True)[:, 0]
+    iso = -np.partition(ws, len(ws) // 2)[len(ws) // 2]
+    return vals.astype(np.float32), np.float32(iso)

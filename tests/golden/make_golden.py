@@ -18,6 +18,10 @@ has the reference tree; the fixtures are committed, the reference is not).
   inputs, the counts, SHA-256 digests of the normalised samples, the leaves
   (depth, offsets in preorder) and the corners (gridDataVector order), and
   the full arrays for the smallest case.
+* pgr_mesh.npz -- the reference's own marching cubes (Octree::initLeaf +
+  GetMCIsoTriangles(iso, mesh, 0, 1, 0, 0) + writePolygon2, same library) on
+  PGR octrees of MESH_CASES: digests of the float32 corner values it was fed,
+  of its vertices, triangles and PLY bytes, and their counts.
 
 Usage:  python tests/golden/make_golden.py [--ref /root/reference]
 """
@@ -113,6 +117,51 @@ def pgr_vectors(bunny_points):
     return out
 
 
+def mesh_cases(bunny):
+    pts = bunny["points"].astype(np.float32)
+    return {
+        # (points, max_depth, min_depth, field, smooth width)
+        "bunny_d8_sphere": (pts, 8, 1, "sphere", 0.005),
+        "bunny_d10_winding": (pts, 10, 1, "winding", 0.005),
+        "bunny_d7_m2_winding": (pts, 7, 2, "winding", 0.005),
+        "torus_d9_sphere": (synth.torus(3000, 51).astype(np.float32), 9, 1, "sphere", 0.004),
+    }
+
+
+def mesh_values(bunny, g, field):
+    if field == "sphere":
+        return oracle.sphere_field(g.corners), np.float32(0.0)
+    return oracle.winding_field(g.samples, bunny["normals"], g.corners)
+
+
+def pgr_mesh_vectors(bunny, tmpdir):
+    out = {}
+    for name, (pts, d, m, field, sw) in mesh_cases(bunny).items():
+        if field == "winding" and len(pts) != len(bunny["normals"]):
+            raise ValueError("winding fields need the bunny's normals")
+        g = oracle.pgr_ref_build(pts, d, m)
+        vals, iso = mesh_values(bunny, g, field)
+        ply = os.path.join(tmpdir, name + ".ply")
+        v, off, idx = oracle.pgr_ref_mesh(vals, iso, sw, ply)
+        assert np.all(np.diff(off) == 3)
+        if not name.startswith("bunny"):
+            out[f"{name}/points"] = pts
+        out[f"{name}/args"] = np.array([d, m], np.int64)
+        out[f"{name}/field"] = np.array(field)
+        out[f"{name}/iso_sw"] = np.array([iso, sw], np.float32)
+        out[f"{name}/counts"] = np.array([len(v), len(off) - 1], np.int64)
+        out[f"{name}/values_sha256"] = np.array(digest(vals))
+        out[f"{name}/vertices_sha256"] = np.array(digest(v))
+        out[f"{name}/triangles_sha256"] = np.array(digest(idx.reshape(-1, 3)))
+        out[f"{name}/ply_sha256"] = np.array(hashlib_file(ply))
+    return out
+
+
+def hashlib_file(path):
+    import hashlib
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
 def bunny(ref_root):
     base = os.path.join(ref_root, "CUDA-based")
     pin = np.loadtxt(os.path.join(base, "input", "bunny.xyz"))
@@ -134,7 +183,10 @@ def main():
     if not oracle.pgr_ref_available():
         sys.exit("oracle/_ref/libpgr_ref.so is missing (make -C oracle needs the reference tree)")
     np.savez_compressed(os.path.join(HERE, "pgr.npz"), **pgr_vectors(bun["points"]))
-    for f in ("ref_core.npz", "bunny.npz", "pgr.npz"):
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        np.savez_compressed(os.path.join(HERE, "pgr_mesh.npz"), **pgr_mesh_vectors(bun, tmp))
+    for f in ("ref_core.npz", "bunny.npz", "pgr.npz", "pgr_mesh.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 

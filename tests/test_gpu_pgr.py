@@ -101,3 +101,65 @@ def test_gauss_recon_values_on_the_pgr_corners():
         # the surface separates the samples' inside from the far corners:
         # the isovalue lies strictly between the values far outside and deep inside
         assert v.min() < iso < v.max()
+
+
+# ---------------------------------------------------------------- marching cubes
+MESH = np.load(os.path.join(GOLD, "pgr_mesh.npz"))
+MESH_CASES = sorted({k.split("/")[0] for k in MESH.files})
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", MESH_CASES)
+def test_pgr_mesh_matches_reference_fixtures(name, tmp_path):
+    """Vertices, triangles and the PLY bytes identical to the reference's own
+    GetMCIsoTriangles + writePolygon2 on the same corner values."""
+    from cagr_amd.pcio import save_ply_mesh
+    d, m = (int(x) for x in MESH[f"{name}/args"])
+    pts = BUNNY["points"].astype(np.float32) if name.startswith("bunny") else MESH[f"{name}/points"]
+    say("pgr mesh", name)
+    o = oracle.pgr_build(pts, d, m)
+    if str(MESH[f"{name}/field"]) == "sphere":
+        vals, iso = oracle.sphere_field(o.corners), np.float32(0.0)
+    else:
+        vals, iso = oracle.winding_field(o.samples, BUNNY["normals"], o.corners)
+    assert _sha(vals) == str(MESH[f"{name}/values_sha256"])
+    sw = float(MESH[f"{name}/iso_sw"][1])
+    g = PGROctree(torch.from_numpy(pts).to(DEV), d, m)
+    v, t = g.mesh(torch.from_numpy(vals).to(DEV), float(iso), sw)
+    assert [v.shape[0], t.shape[0]] == MESH[f"{name}/counts"].tolist()
+    assert _sha(v.cpu().numpy()) == str(MESH[f"{name}/vertices_sha256"])
+    assert _sha(t.cpu().numpy()) == str(MESH[f"{name}/triangles_sha256"])
+    path = str(tmp_path / "m.ply")
+    save_ply_mesh(path, g.output_vertices(v), t.cpu().numpy())
+    assert _sha(np.frombuffer(open(path, "rb").read(), np.uint8)) == str(MESH[f"{name}/ply_sha256"])
+    # a second mesh on the same octree replaces the first and is the same
+    v2, t2 = g.mesh(torch.from_numpy(vals).to(DEV), float(iso), sw)
+    assert torch.equal(v, v2) and torch.equal(t, t2)
+
+
+def test_gauss_recon_end_to_end(tmp_path):
+    """main_GaussRecon_cuda.cu as one call on the bunny's oriented samples: the
+    FP64 field differs from the oracle's only in summation order, so the mesh
+    matches the reference's on the oracle's field in size and lies within a
+    float32 rounding of it."""
+    from cagr_amd.pgr import gauss_recon
+    say("gauss recon end to end")
+    pts = torch.from_numpy(BUNNY["points"].astype(np.float32)).to(DEV)
+    nml = torch.from_numpy(BUNNY["normals"]).to(DEV)
+    path = str(tmp_path / "bunny.ply")
+    v, t, iso = gauss_recon(pts, nml, path, max_depth=10, precision="fp64")
+    nv, nt = MESH["bunny_d10_winding/counts"].tolist()
+    assert abs(len(v) - nv) <= 2 and abs(len(t) - nt) <= 4
+    assert abs(iso - float(MESH["bunny_d10_winding/iso_sw"][0])) <= 1e-6
+    data = open(path, "rb").read()
+    assert data.startswith(b"ply\nformat binary_little_endian 1.0\nelement vertex %d\n" % len(v))
+    lo, hi = BUNNY["points"].min(0), BUNNY["points"].max(0)
+    span = (hi - lo).max()
+    assert np.all(v >= lo - 0.1 * span) and np.all(v <= hi + 0.1 * span)
+    # FP32c: the same surface up to the operator's tolerance
+    v32, t32, _ = gauss_recon(pts, nml, None, max_depth=10, precision="fp32c")
+    assert abs(len(v32) - nv) <= 0.01 * nv
