@@ -36,8 +36,9 @@ grid query), c4 (500k displaced sphere: autograd forward + backward, FP64
 finite-difference check on 64 entries), c5 (64 clouds, ~3M points: autotuned
 batched persistent launch against one launch per cloud), pgr (GaussRecon on
 the 1M-point ellipsoid cloud: the PGR octree query set built on the GPU
-against the reference's own Octree::setTree on a host core, then the A3
-winding numbers at its corners and samples).
+against the reference's own Octree::setTree on a host core, the A3 winding
+numbers at its corners and samples, and marching cubes on the octree against
+the reference's own GetMCIsoTriangles).
 """
 from __future__ import annotations
 
@@ -640,7 +641,7 @@ def config_pgr(args):
     build_s = float(np.median(builds))
     nc, nl = oct_.corners.shape[0], oct_.leaves.shape[0]
     log(f"pgr: GPU build {build_s * 1e3:.1f} ms, {nl} leaves, {nc} corners, depth {oct_.depth}")
-    out = {"metric": "PGR query-set build seconds; A3 values at its corners (queries/s)",
+    out = {"metric": "PGR query-set build seconds; A3 values at its corners (queries/s); mesh seconds",
            "config": {"workload": f"GaussRecon on {n} ellipsoid points, -d {D} -m {M} -w {width}, "
                                   "leaf-1 treecode, theta 0.5", "points": n},
            "data": "synthetic",
@@ -667,6 +668,14 @@ def config_pgr(args):
             out["fp32c"]["corner_err_vs_fp64"] = float((v32 - vals.double()).abs().max()
                                                        / vals.double().abs().max())
         log(f"pgr {prec}: corners {qs * 1e3:.1f} ms, samples {ss * 1e3:.1f} ms, iso {iso:.4g}")
+        if prec == "fp32c":
+            vals32, iso32 = (-vals[:, 0]).float().contiguous(), iso
+            oct_.mesh(vals32, iso32, width / 2)
+            ms, (mv, mt) = _sync_time(lambda: oct_.mesh(vals32, iso32, width / 2))
+            out["mesh"] = {"seconds": round(ms, 4), "vertices": int(mv.shape[0]),
+                           "triangles": int(mt.shape[0]),
+                           "field": "FP32c -wn at the corners, its isovalue, smoothWidth w/2"}
+            log(f"pgr mesh: {ms * 1e3:.1f} ms, {mv.shape[0]} vertices, {mt.shape[0]} triangles")
     if not args.no_cpu:
         kind = "reference" if oracle.pgr_ref_available() else "port"
         fn = oracle.pgr_ref_build if kind == "reference" else oracle.pgr_build
@@ -676,6 +685,17 @@ def config_pgr(args):
         same = bool(np.array_equal(ref.corners, oct_.corners.cpu().numpy())
                     and np.array_equal(ref.leaves, oct_.leaves.cpu().numpy()))
         log(f"pgr: CPU {kind} build {cpu_s:.2f} s, identical query set: {same}")
+        if kind == "reference" and "mesh" in out:
+            t = time.perf_counter()
+            rv, roff, ridx = oracle.pgr_ref_mesh(vals32.cpu().numpy(), iso32, width / 2)
+            mesh_cpu = time.perf_counter() - t
+            out["mesh"]["cpu_reference_s"] = round(mesh_cpu, 3)
+            out["mesh"]["identical_to_cpu"] = bool(
+                np.array_equal(rv, mv.cpu().numpy())
+                and np.array_equal(ridx.reshape(-1, 3), mt.cpu().numpy()))
+            out["mesh"]["speedup_vs_cpu"] = round(mesh_cpu / out["mesh"]["seconds"], 1)
+            log(f"pgr: CPU reference mesh {mesh_cpu:.2f} s, identical: "
+                f"{out['mesh']['identical_to_cpu']}")
         out["cpu_baseline"] = {"value": round(cpu_s, 3), "unit": "s (PGR build)", "cores": 1,
                                "kind": kind,
                                "sample": "the whole build on the same 1M points"

@@ -113,6 +113,56 @@ __global__ __launch_bounds__(kT) void k_mc_vertices(View t, const int32_t* __res
     v[2] = p[2];
 }
 
+// left-aligned Morton key of a cell at the tree height H (x lowest): the
+// leaves in preorder are sorted by it
+__device__ inline uint64_t morton_left(const Node& n, int H) {
+    uint64_t m = 0;
+    for (int b = 0; b < n.d; ++b)
+        for (int a = 0; a < 3; ++a)
+            m |= (uint64_t)((n.o[a] >> (n.d - 1 - b)) & 1) << (3 * (H - 1 - b) + a);
+    return m;
+}
+
+__global__ __launch_bounds__(kT) void k_leaf_morton(const int32_t* __restrict__ leaves, int64_t nl,
+                                                    int H, uint64_t* __restrict__ mk) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r < nl) mk[r] = morton_left(leaf_node(leaves, r), H);
+}
+
+// leaves that can emit triangles: those with roots, and the coarser leaf
+// covering each face-neighbour cell of one (GetMCIsoEdges gathers a coarse
+// leaf's edges from the finer leaves across its faces); a superset is fine
+__global__ __launch_bounds__(kT) void k_mc_active(View t, const int32_t* __restrict__ leaves,
+                                                  int64_t nl, int H,
+                                                  const uint64_t* __restrict__ mk,
+                                                  const int32_t* __restrict__ valid,
+                                                  int32_t* __restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= nl) return;
+    bool roots = false;
+    for (int s = 0; s < 12; ++s) roots = roots || valid[12 * r + s + 1] != valid[12 * r + s];
+    if (!roots) return;
+    active[r] = 1;
+    const Node n = leaf_node(leaves, r);
+    const int64_t lim = (int64_t)1 << n.d;
+    for (int f = 0; f < 6; ++f) {
+        Node c = n;
+        c.o[f >> 1] += (f & 1) ? 1 : -1;
+        if (c.o[f >> 1] < 0 || c.o[f >> 1] >= lim) continue;
+        // the deepest existing ancestor-or-self of the neighbour cell
+        while (c.d > 0 && !is_internal(t, parent(c))) c = parent(c);
+        if (c.d == n.d) continue;                       // same depth: not coarser
+        const int64_t i = lower_bound(mk, nl, morton_left(c, H));
+        if (i < nl) active[i] = 1;
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_compact_ids(const int32_t* __restrict__ pos, int64_t n,
+                                                    int32_t* __restrict__ ids) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r < n && pos[r + 1] != pos[r]) ids[pos[r]] = (int32_t)r;
+}
+
 struct DevVIndex {
     const int64_t* vkeys;
     const int32_t* vidx;
@@ -131,7 +181,8 @@ struct DevVIndex {
 // workspace per thread, leaves by grid stride
 template <bool kCountOnly>
 __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restrict__ leaves,
-                                                int64_t nl, DevVIndex vi, Work* __restrict__ ws,
+                                                const int32_t* __restrict__ ids, int64_t na,
+                                                DevVIndex vi, Work* __restrict__ ws,
                                                 int32_t* __restrict__ counts,
                                                 const int64_t* __restrict__ offs,
                                                 int32_t* __restrict__ tris,
@@ -139,8 +190,9 @@ __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restric
     const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * 64;
     Work& w = ws[tid];
-    for (int64_t r = tid; r < nl; r += nthreads) {
-        int32_t* out = kCountOnly ? nullptr : tris + 3 * offs[r];
+    for (int64_t a = tid; a < na; a += nthreads) {
+        const int64_t r = ids[a];
+        int32_t* out = kCountOnly ? nullptr : tris + 3 * offs[a];
         int k = 0;
         const int c = leaf_triangles<kCountOnly>(t, leaf_node(leaves, r), w, vi,
                                                  [&](int a, int b, int cc) {
@@ -151,10 +203,10 @@ __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restric
                                                  });
         if (c < 0) {
             atomicOr(err, 1);
-            if (kCountOnly) counts[r] = 0;
+            if (kCountOnly) counts[a] = 0;
             continue;
         }
-        if (kCountOnly) counts[r] = c;
+        if (kCountOnly) counts[a] = c;
     }
 }
 
@@ -237,37 +289,62 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     k_mc_vkeys<<<nblk(m), kT, 0, st>>>(keys.p, occ.p, m, head.p, first.p, vkeys.p, vidx.p);
     CAGR_LAUNCHED();
 
-    // triangles: count, offsets, write
-    const int64_t nthreads = std::min<int64_t>(nl, 256 * 64 * 2);
+    // triangles: the leaves that can emit (compacted, in preorder), counted,
+    // given offsets by a scan, then written
+    const int H = g->depth - 1;
+    Buf<uint64_t> mk;
+    Buf<int32_t> active;
+    if ((rc = mk.alloc(nl)) || (rc = active.alloc(nl + 1))) return rc;
+    k_leaf_morton<<<nblk(nl), kT, 0, st>>>(g->leaves, nl, H, mk.p);
+    CAGR_LAUNCHED();
+    k_fill<<<nblk(nl + 1), kT, 0, st>>>(active.p, nl + 1);
+    CAGR_LAUNCHED();
+    k_mc_active<<<nblk(nl), kT, 0, st>>>(t, g->leaves, nl, H, mk.p, valid.p, active.p);
+    CAGR_LAUNCHED();
+    Buf<int32_t> atmp;
+    if ((rc = atmp.alloc(scan_tmp_len(nl + 1)))) return rc;
+    if ((rc = scan_exclusive<int32_t>(active.p, nl + 1, atmp.p, st))) return rc;
+    int32_t na32 = 0;
+    CAGR_HIP(hipMemcpyAsync(&na32, active.p + nl, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    const int64_t na = na32;
+    Buf<int32_t> ids;
+    if ((rc = ids.alloc(na))) return rc;
+    k_compact_ids<<<nblk(nl), kT, 0, st>>>(active.p, nl, ids.p);
+    CAGR_LAUNCHED();
+
+    // one workspace per thread; enough threads to keep every CU busy
+    const int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(na, 256 * 64 * 16));
     const int64_t nb = (nthreads + 63) / 64;
     Buf<Work> ws;
     Buf<int32_t> cnt, err;
     Buf<int64_t> offs;
-    if ((rc = ws.alloc(nb * 64)) || (rc = cnt.alloc(nl + 1)) || (rc = err.alloc(1)) ||
-        (rc = offs.alloc(nl + 1)))
+    if ((rc = ws.alloc(nb * 64)) || (rc = cnt.alloc(na + 1)) || (rc = err.alloc(1)) ||
+        (rc = offs.alloc(na + 1)))
         return rc;
     k_fill<<<1, kT, 0, st>>>(err.p, 1);
     CAGR_LAUNCHED();
     const DevVIndex vi{vkeys.p, vidx.p, nv, g->mesh_v};
-    k_mc_tris<true><<<nb, 64, 0, st>>>(t, g->leaves, nl, vi, ws.p, cnt.p, nullptr, nullptr, err.p);
+    k_mc_tris<true><<<nb, 64, 0, st>>>(t, g->leaves, ids.p, na, vi, ws.p, cnt.p, nullptr, nullptr,
+                                        err.p);
     CAGR_LAUNCHED();
-    k_fill<<<1, kT, 0, st>>>(cnt.p + nl, 1);
+    k_fill<<<1, kT, 0, st>>>(cnt.p + na, 1);
     CAGR_LAUNCHED();
-    k_widen<<<nblk(nl + 1), kT, 0, st>>>(cnt.p, nl + 1, offs.p);
+    k_widen<<<nblk(na + 1), kT, 0, st>>>(cnt.p, na + 1, offs.p);
     CAGR_LAUNCHED();
     Buf<int64_t> otmp64;
-    if ((rc = otmp64.alloc(scan_tmp_len(nl + 1)))) return rc;
-    if ((rc = scan_exclusive<int64_t>(offs.p, nl + 1, otmp64.p, st))) return rc;
+    if ((rc = otmp64.alloc(scan_tmp_len(na + 1)))) return rc;
+    if ((rc = scan_exclusive<int64_t>(offs.p, na + 1, otmp64.p, st))) return rc;
     int64_t nt = 0;
     int32_t e = 0;
-    CAGR_HIP(hipMemcpyAsync(&nt, offs.p + nl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipMemcpyAsync(&nt, offs.p + na, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipMemcpyAsync(&e, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
     CAGR_REQUIRE(e == 0, CAGR_ERR_STATE,
                  "a leaf exceeded the marching-cubes workspace (128 iso edges or roots)");
     CAGR_HIP(hipMalloc(&g->mesh_t, 3 * std::max<int64_t>(nt, 1) * sizeof(int32_t)));
-    k_mc_tris<false><<<nb, 64, 0, st>>>(t, g->leaves, nl, vi, ws.p, nullptr, offs.p, g->mesh_t,
-                                         err.p);
+    k_mc_tris<false><<<nb, 64, 0, st>>>(t, g->leaves, ids.p, na, vi, ws.p, nullptr, offs.p,
+                                         g->mesh_t, err.p);
     CAGR_LAUNCHED();
     CAGR_HIP(hipStreamSynchronize(st));
     g->n_vertices = nv;
