@@ -39,7 +39,8 @@ namespace cagr {
 namespace mc {
 
 // limits of the per-leaf work (checked; exceeding them is reported)
-// (test clouds peak at 19 edges and 19 roots per leaf)
+// (the test clouds peak at 19 edges and 19 roots per leaf; the 1M-point
+// ellipsoid cloud needs more than 64 and fits 128)
 constexpr int kMaxEdges = 128;     // iso edges of one leaf incl. finer neighbours
 constexpr int kMaxVerts = 128;     // distinct roots met by one leaf
 constexpr int kMaxWalk = 48;       // explicit stacks of the descents (depth <= 13)
@@ -312,6 +313,9 @@ struct Work {
     int tt_e[kMaxEdges][3];
     int nte, ntt;
     bool overflow;
+    // the leaf's own 12 edge roots, found on first use (0 unknown, 1 found, 2 none)
+    int8_t own_state[12];
+    RootInfo own[12];
 };
 
 CAGR_HD inline int vfind(const Work& w, int64_t key) {
@@ -357,10 +361,17 @@ CAGR_HD inline void face_edges_of(const View& t, const Node& n1, int f, Work& w)
         }
 }
 
+CAGR_HD inline bool own_root(const View& t, const Node& n, int e, Work& w, RootInfo& ri) {
+    if (w.own_state[e] == 0) w.own_state[e] = root_index(t, n, e, w.own[e]) ? 1 : 2;
+    ri = w.own[e];
+    return w.own_state[e] == 1;
+}
+
 // Octree::GetMCIsoEdges (Octree.cpp:900-1006)
 CAGR_HD inline void iso_edges(const View& t, const Node& n, Work& w) {
     w.ne = 0;
     w.nv = 0;
+    for (int e = 0; e < 12; ++e) w.own_state[e] = 0;
     const int mc = mc_index(t, n);
     const int8_t* tri = kTriTable[mc];
     for (int f = 0; f < 6; ++f) {
@@ -395,7 +406,7 @@ CAGR_HD inline void iso_edges(const View& t, const Node& n, Work& w) {
                     const int e1 = tri[j + k], e2 = tri[j + (k + 1) % 3];
                     if (face_of_edges(e1, e2) != f) continue;
                     RootInfo r1, r2;
-                    if (root_index(t, n, e1, r1) && root_index(t, n, e2, r2)) {
+                    if (own_root(t, n, e1, w, r1) && own_root(t, n, e2, w, r2)) {
                         push_edge(w, r1.key, r2.key);
                         const int i1 = vget(w, r1), i2 = vget(w, r2);
                         if (i1 < 0 || i2 < 0) return;

@@ -314,7 +314,7 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     CAGR_LAUNCHED();
 
     // one workspace per thread; enough threads to keep every CU busy
-    const int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(na, 256 * 64 * 16));
+    const int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(na, 256 * 64 * 12));
     const int64_t nb = (nthreads + 63) / 64;
     Buf<Work> ws;
     Buf<int32_t> cnt, err;
