@@ -62,6 +62,12 @@ struct View {
     float iso;
     float smooth_width;
     float grid;                    // 1 / 2^(md + 1): coordinate of one key unit
+    // optional: marching-cubes indices computed once per node (null: from
+    // the corner values at every use)
+    const uint8_t* in_mc;          // per internal node, in in_keys order
+    const uint64_t* leaf_mk;       // leaves' left-aligned Morton keys (preorder = ascending)
+    const uint8_t* leaf_mc;        // per leaf
+    int64_t n_leaves;
 };
 
 struct RootInfo {
@@ -169,8 +175,28 @@ CAGR_HD inline int32_t corner_slot(const View& t, const Node& n, int c) {
     const int64_t i = lower_bound(t.ck, t.n_ck, k);
     return (i < t.n_ck && t.ck[i] == k) ? t.ck_idx[i] : -1;
 }
+CAGR_HD inline uint64_t morton_left(const Node& n, int H) {
+    uint64_t m = 0;
+    for (int b = 0; b < n.d; ++b)
+        for (int a = 0; a < 3; ++a)
+            m |= (uint64_t)((n.o[a] >> (n.d - 1 - b)) & 1) << (3 * (H - 1 - b) + a);
+    return m;
+}
+
 // MarchingCubes::GetIndex (MarchingCubes.cpp:472-483) at the node's corners
+CAGR_HD inline int mc_from_corners(const View& t, const Node& n);
 CAGR_HD inline int mc_index(const View& t, const Node& n) {
+    if (t.in_mc) {
+        const uint64_t k = node_key(n.d, n.o[0], n.o[1], n.o[2]);
+        const int64_t i = lower_bound(t.in_keys, t.n_in, k);
+        if (i < t.n_in && t.in_keys[i] == k) return t.in_mc[i];
+        const uint64_t m = morton_left(n, t.md - 1);
+        const int64_t j = lower_bound(t.leaf_mk, t.n_leaves, m);
+        if (j < t.n_leaves && t.leaf_mk[j] == m) return t.leaf_mc[j];
+    }
+    return mc_from_corners(t, n);
+}
+CAGR_HD inline int mc_from_corners(const View& t, const Node& n) {
     int idx = 0;
     for (int c = 0; c < 8; ++c) {
         const int32_t s = corner_slot(t, n, c);

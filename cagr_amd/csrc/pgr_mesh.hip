@@ -113,20 +113,25 @@ __global__ __launch_bounds__(kT) void k_mc_vertices(View t, const int32_t* __res
     v[2] = p[2];
 }
 
-// left-aligned Morton key of a cell at the tree height H (x lowest): the
-// leaves in preorder are sorted by it
-__device__ inline uint64_t morton_left(const Node& n, int H) {
-    uint64_t m = 0;
-    for (int b = 0; b < n.d; ++b)
-        for (int a = 0; a < 3; ++a)
-            m |= (uint64_t)((n.o[a] >> (n.d - 1 - b)) & 1) << (3 * (H - 1 - b) + a);
-    return m;
+// the leaves' left-aligned Morton keys (ascending: preorder) and every
+// node's marching-cubes index from its corner values, once
+__global__ __launch_bounds__(kT) void k_leaf_morton(View t, const int32_t* __restrict__ leaves,
+                                                    int64_t nl, int H, uint64_t* __restrict__ mk,
+                                                    uint8_t* __restrict__ mc) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= nl) return;
+    const Node n = leaf_node(leaves, r);
+    mk[r] = morton_left(n, H);
+    mc[r] = (uint8_t)mc_from_corners(t, n);
 }
 
-__global__ __launch_bounds__(kT) void k_leaf_morton(const int32_t* __restrict__ leaves, int64_t nl,
-                                                    int H, uint64_t* __restrict__ mk) {
+__global__ __launch_bounds__(kT) void k_inner_mc(View t, int64_t n_in, uint8_t* __restrict__ mc) {
     const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
-    if (r < nl) mk[r] = morton_left(leaf_node(leaves, r), H);
+    if (r >= n_in) return;
+    const uint64_t k = t.in_keys[r];
+    const Node n{(int)(k >> 39), {(int32_t)(k & 0x1FFF), (int32_t)((k >> 13) & 0x1FFF),
+                                  (int32_t)((k >> 26) & 0x1FFF)}};
+    mc[r] = (uint8_t)mc_from_corners(t, n);
 }
 
 // leaves that can emit triangles: those with roots, and the coarser leaf
@@ -243,6 +248,20 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     t.grid = (float)(1.0f / std::pow(2.0, t.md + 1));
     const int64_t nl = g->n_leaves, n12 = 12 * nl;
     CAGR_REQUIRE(n12 < (int64_t(1) << 31), CAGR_ERR_ARG, "too many leaves for the mesh");
+    const int H = g->depth - 1;
+    Buf<uint64_t> mk;
+    Buf<uint8_t> leaf_mc, in_mc;
+    if ((rc = mk.alloc(nl)) || (rc = leaf_mc.alloc(nl)) || (rc = in_mc.alloc(g->n_in))) return rc;
+    k_leaf_morton<<<nblk(nl), kT, 0, st>>>(t, g->leaves, nl, H, mk.p, leaf_mc.p);
+    CAGR_LAUNCHED();
+    if (g->n_in) {
+        k_inner_mc<<<nblk(g->n_in), kT, 0, st>>>(t, g->n_in, in_mc.p);
+        CAGR_LAUNCHED();
+    }
+    t.in_mc = in_mc.p;
+    t.leaf_mk = mk.p;
+    t.leaf_mc = leaf_mc.p;
+    t.n_leaves = nl;
 
     // vertices: root keys per (leaf, slot), the valid ones compacted and
     // sorted stably by key
@@ -291,12 +310,8 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
 
     // triangles: the leaves that can emit (compacted, in preorder), counted,
     // given offsets by a scan, then written
-    const int H = g->depth - 1;
-    Buf<uint64_t> mk;
     Buf<int32_t> active;
-    if ((rc = mk.alloc(nl)) || (rc = active.alloc(nl + 1))) return rc;
-    k_leaf_morton<<<nblk(nl), kT, 0, st>>>(g->leaves, nl, H, mk.p);
-    CAGR_LAUNCHED();
+    if ((rc = active.alloc(nl + 1))) return rc;
     k_fill<<<nblk(nl + 1), kT, 0, st>>>(active.p, nl + 1);
     CAGR_LAUNCHED();
     k_mc_active<<<nblk(nl), kT, 0, st>>>(t, g->leaves, nl, H, mk.p, valid.p, active.p);

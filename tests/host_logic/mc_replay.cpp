@@ -62,6 +62,24 @@ extern "C" int hl_mesh(const int32_t* leaves, int64_t nl, int md, const float* v
     t.iso = iso;
     t.smooth_width = smooth_width;
     t.grid = (float)(1.0f / std::pow(2.0, md + 1));
+    // the per-node marching-cubes indices the kernels precompute
+    std::vector<uint64_t> mk(nl);
+    std::vector<uint8_t> leaf_mc(nl), in_mc(in.size());
+    for (int64_t r = 0; r < nl; ++r) {
+        Node n{leaves[4 * r], {leaves[4 * r + 1], leaves[4 * r + 2], leaves[4 * r + 3]}};
+        mk[r] = morton_left(n, md - 1);
+        leaf_mc[r] = (uint8_t)mc_from_corners(t, n);
+    }
+    for (size_t i = 0; i < in.size(); ++i) {
+        const uint64_t k = in[i];
+        Node n{(int)(k >> 39), {(int32_t)(k & 0x1FFF), (int32_t)((k >> 13) & 0x1FFF),
+                                (int32_t)((k >> 26) & 0x1FFF)}};
+        in_mc[i] = (uint8_t)mc_from_corners(t, n);
+    }
+    t.in_mc = in_mc.data();
+    t.leaf_mk = mk.data();
+    t.leaf_mc = leaf_mc.data();
+    t.n_leaves = nl;
 
     // vertices
     std::map<int64_t, int32_t> vid;
