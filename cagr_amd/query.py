@@ -8,10 +8,11 @@ Restates the query half of CUDA-based/ext/gaussrecon_src/main_GaussRecon_cuda.cu
   the samples themselves (:450-467);
 * value = -wn at every query, smoothWidth = width / 2 (:477-480); isovalue =
   -wn at the samples, element N/2 after nth_element (:485-486).
-The reference queries the corners of its adaptive PGR octree; here the query
-set is a regular grid (generated inside the kernel) or any caller-given
-points.  The points are expected in the [-1, 1]^3 convention (the reference
-maps its [0,1]^3 PGR coordinates with 2p - 1).
+The reference queries the corners of its adaptive PGR octree, which
+``cagr_amd.pgr`` builds (and meshes); the functions here take a regular grid
+(generated inside the kernel) or any caller-given points.  The points are
+expected in the [-1, 1]^3 convention (the reference maps its [0,1]^3 PGR
+coordinates with 2p - 1).
 """
 from __future__ import annotations
 
