@@ -342,6 +342,7 @@ struct Work {
     // the leaf's own 12 edge roots, found on first use (0 unknown, 1 found, 2 none)
     int8_t own_state[12];
     RootInfo own[12];
+    int32_t out[3 * kMaxEdges];                // the leaf's triangles, before they are placed
 };
 
 CAGR_HD inline int vfind(const Work& w, int64_t key) {

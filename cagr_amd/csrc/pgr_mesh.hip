@@ -8,10 +8,11 @@
 // stably, the first occurrence of each key over (leaf preorder, slot) gets
 // its vertex number by a scan -- the order the reference's roots map fills
 // in -- and that occurrence computes the position.
-// Triangles: a count pass and a write pass over the leaves (persistent
-// threads, one workspace each in global memory) give every leaf its loops,
-// the loops their fans and area-minimising flips, at offsets from a scan,
-// so the triangle list comes out in the reference's order.
+// Triangles: one pass over the leaves that can emit (persistent threads, one
+// workspace each in global memory) gives every leaf its loops and the loops
+// their fans and area-minimising flips; each leaf reserves room for its run
+// with one atomic add, and a scan of the per-leaf counts then moves every
+// run to its place, so the triangle list comes out in the reference's order.
 #include <algorithm>
 
 #include "common.h"
@@ -182,37 +183,57 @@ struct DevVIndex {
     }
 };
 
-// count (tris == nullptr) or write the triangles of every leaf; one
-// workspace per thread, leaves by grid stride
-template <bool kCountOnly>
+// the triangles of every leaf that can emit, in one pass: each leaf builds
+// them in its workspace, reserves room in `tmp` with one atomic add and
+// copies them there; counts and reservations are kept per leaf so that
+// k_mc_place can move every leaf's run to its place in preorder.  Threads
+// persist over the leaves (grid stride), one workspace each.
 __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restrict__ leaves,
                                                 const int32_t* __restrict__ ids, int64_t na,
                                                 DevVIndex vi, Work* __restrict__ ws,
                                                 int32_t* __restrict__ counts,
-                                                const int64_t* __restrict__ offs,
-                                                int32_t* __restrict__ tris,
+                                                int64_t* __restrict__ base,
+                                                unsigned long long* __restrict__ used,
+                                                int64_t cap, int32_t* __restrict__ tmp,
                                                 int32_t* __restrict__ err) {
     const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * 64;
     Work& w = ws[tid];
     for (int64_t a = tid; a < na; a += nthreads) {
-        const int64_t r = ids[a];
-        int32_t* out = kCountOnly ? nullptr : tris + 3 * offs[a];
         int k = 0;
-        const int c = leaf_triangles<kCountOnly>(t, leaf_node(leaves, r), w, vi,
-                                                 [&](int a, int b, int cc) {
-                                                     out[3 * k] = a;
-                                                     out[3 * k + 1] = b;
-                                                     out[3 * k + 2] = cc;
-                                                     ++k;
-                                                 });
-        if (c < 0) {
+        const int c = leaf_triangles<false>(t, leaf_node(leaves, ids[a]), w, vi,
+                                            [&](int x, int y, int z) {
+                                                if (k < kMaxEdges) {
+                                                    w.out[3 * k] = x;
+                                                    w.out[3 * k + 1] = y;
+                                                    w.out[3 * k + 2] = z;
+                                                }
+                                                ++k;
+                                            });
+        if (c < 0 || c > kMaxEdges) {
             atomicOr(err, 1);
-            if (kCountOnly) counts[a] = 0;
+            counts[a] = 0;
+            base[a] = 0;
             continue;
         }
-        if (kCountOnly) counts[a] = c;
+        counts[a] = c;
+        const int64_t b = c ? (int64_t)atomicAdd(used, (unsigned long long)c) : 0;
+        base[a] = b;
+        if (b + c <= cap)
+            for (int q = 0; q < 3 * c; ++q) tmp[3 * b + q] = w.out[q];
     }
+}
+
+// every leaf's run of triangles from its reservation to its place in preorder
+__global__ __launch_bounds__(kT) void k_mc_place(const int32_t* __restrict__ counts,
+                                                 const int64_t* __restrict__ base,
+                                                 const int64_t* __restrict__ offs, int64_t na,
+                                                 const int32_t* __restrict__ tmp,
+                                                 int32_t* __restrict__ tris) {
+    const int64_t a = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (a >= na) return;
+    const int64_t from = 3 * base[a], to = 3 * offs[a];
+    for (int q = 0; q < 3 * counts[a]; ++q) tris[to + q] = tmp[from + q];
 }
 
 __global__ __launch_bounds__(kT) void k_widen(const int32_t* __restrict__ c, int64_t n,
@@ -333,16 +354,39 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     const int64_t nb = (nthreads + 63) / 64;
     Buf<Work> ws;
     Buf<int32_t> cnt, err;
-    Buf<int64_t> offs;
+    Buf<int64_t> base, offs;
+    Buf<unsigned long long> used;
     if ((rc = ws.alloc(nb * 64)) || (rc = cnt.alloc(na + 1)) || (rc = err.alloc(1)) ||
-        (rc = offs.alloc(na + 1)))
+        (rc = base.alloc(na)) || (rc = offs.alloc(na + 1)) || (rc = used.alloc(1)))
         return rc;
-    k_fill<<<1, kT, 0, st>>>(err.p, 1);
-    CAGR_LAUNCHED();
     const DevVIndex vi{vkeys.p, vidx.p, nv, g->mesh_v};
-    k_mc_tris<true><<<nb, 64, 0, st>>>(t, g->leaves, ids.p, na, vi, ws.p, cnt.p, nullptr, nullptr,
-                                        err.p);
-    CAGR_LAUNCHED();
+    // a closed surface has about two triangles per vertex (the 1M-point
+    // benchmark mesh: 2.003); a larger mesh runs the pass again with the
+    // exact size
+    int64_t cap = 2 * (int64_t)nv + (int64_t)nv / 10 + 65536;
+    Buf<int32_t> tmp;
+    for (int attempt = 0;; ++attempt) {
+        if ((rc = tmp.alloc(3 * cap))) return rc;
+        k_fill<<<1, kT, 0, st>>>(err.p, 1);
+        CAGR_LAUNCHED();
+        k_fill<<<1, kT, 0, st>>>((int32_t*)used.p, 2);
+        CAGR_LAUNCHED();
+        k_mc_tris<<<nb, 64, 0, st>>>(t, g->leaves, ids.p, na, vi, ws.p, cnt.p, base.p, used.p,
+                                     cap, tmp.p, err.p);
+        CAGR_LAUNCHED();
+        unsigned long long total = 0;
+        int32_t e = 0;
+        CAGR_HIP(hipMemcpyAsync(&total, used.p, sizeof(total), hipMemcpyDeviceToHost, st));
+        CAGR_HIP(hipMemcpyAsync(&e, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CAGR_HIP(hipStreamSynchronize(st));
+        CAGR_REQUIRE(e == 0, CAGR_ERR_STATE,
+                     "a leaf exceeded the marching-cubes workspace (128 iso edges or roots)");
+        if ((int64_t)total <= cap) break;
+        CAGR_REQUIRE(attempt == 0, CAGR_ERR_STATE, "mesh size changed between two passes");
+        CAGR_HIP(hipFree(tmp.p));
+        tmp.p = nullptr;
+        cap = (int64_t)total;
+    }
     k_fill<<<1, kT, 0, st>>>(cnt.p + na, 1);
     CAGR_LAUNCHED();
     k_widen<<<nblk(na + 1), kT, 0, st>>>(cnt.p, na + 1, offs.p);
@@ -351,15 +395,10 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     if ((rc = otmp64.alloc(scan_tmp_len(na + 1)))) return rc;
     if ((rc = scan_exclusive<int64_t>(offs.p, na + 1, otmp64.p, st))) return rc;
     int64_t nt = 0;
-    int32_t e = 0;
     CAGR_HIP(hipMemcpyAsync(&nt, offs.p + na, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    CAGR_HIP(hipMemcpyAsync(&e, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
-    CAGR_REQUIRE(e == 0, CAGR_ERR_STATE,
-                 "a leaf exceeded the marching-cubes workspace (128 iso edges or roots)");
     CAGR_HIP(hipMalloc(&g->mesh_t, 3 * std::max<int64_t>(nt, 1) * sizeof(int32_t)));
-    k_mc_tris<false><<<nb, 64, 0, st>>>(t, g->leaves, ids.p, na, vi, ws.p, nullptr, offs.p,
-                                         g->mesh_t, err.p);
+    k_mc_place<<<nblk(na), kT, 0, st>>>(cnt.p, base.p, offs.p, na, tmp.p, g->mesh_t);
     CAGR_LAUNCHED();
     CAGR_HIP(hipStreamSynchronize(st));
     g->n_vertices = nv;
