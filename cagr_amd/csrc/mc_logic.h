@@ -626,7 +626,7 @@ namespace mc {
 // from its first vertex, flipped to minimum area; `vindex` maps a root key to
 // its vertex number (-1 if none: the reference skips such a vertex).  Emits
 // (a, b, c) through `emit` in the reference's order; returns the count.
-template <bool kCountOnly, class VIndex, class Emit>
+template <class VIndex, class Emit>
 CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VIndex vindex,
                                   Emit emit) {
     w.overflow = false;
@@ -640,10 +640,6 @@ CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VInd
         for (int j = 0; j < len; ++j) {
             const int v = vindex(w.loop[j]);
             if (v >= 0) vid[nv++] = v;
-        }
-        if (kCountOnly) {
-            if (nv >= 3) count += nv - 2;    // a fan of nv-2 triangles, flips keep the count
-            continue;
         }
         if (nv > 3) {
             w.nte = w.ntt = 0;

@@ -201,7 +201,7 @@ __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restric
     Work& w = ws[tid];
     for (int64_t a = tid; a < na; a += nthreads) {
         int k = 0;
-        const int c = leaf_triangles<false>(t, leaf_node(leaves, ids[a]), w, vi,
+        const int c = leaf_triangles(t, leaf_node(leaves, ids[a]), w, vi,
                                             [&](int x, int y, int z) {
                                                 if (k < kMaxEdges) {
                                                     w.out[3 * k] = x;

@@ -113,7 +113,7 @@ extern "C" int hl_mesh(const int32_t* leaves, int64_t nl, int md, const float* v
     auto w = std::make_unique<Work>();
     for (int64_t r = 0; r < nl; ++r) {
         Node n{leaves[4 * r], {leaves[4 * r + 1], leaves[4 * r + 2], leaves[4 * r + 3]}};
-        const int c = leaf_triangles<false>(t, n, *w, vi, [&](int a, int b, int cc) {
+        const int c = leaf_triangles(t, n, *w, vi, [&](int a, int b, int cc) {
             g_tri.push_back(a);
             g_tri.push_back(b);
             g_tri.push_back(cc);
