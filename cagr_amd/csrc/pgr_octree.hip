@@ -332,17 +332,30 @@ __device__ inline void splat_recs(const float q[3], int d, float len, uint64_t* 
         }
 }
 
+// Only the centre weights of depths >= min_level are read: a sample whose
+// depth search climbs above them is deferred (todo[i] = 1, no records
+// written) to a second launch with every level; `retry` launches process
+// only the samples with todo set.
 __global__ __launch_bounds__(kT) void k_splat_recs(const float* __restrict__ p, int64_t n, int S,
                                                    int D, int dmin, double log4, float normal_len,
-                                                   Levels L, uint64_t* __restrict__ keys,
+                                                   Levels L, int min_level, bool retry,
+                                                   int32_t* __restrict__ todo,
+                                                   int32_t* __restrict__ n_todo,
+                                                   uint64_t* __restrict__ keys,
                                                    int32_t* __restrict__ vals,
                                                    double* __restrict__ v64) {
     const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
+    if (retry && !todo[i]) return;
+    auto defer = [&]() {
+        todo[i] = 1;
+        atomicAdd(n_todo, 1);
+    };
     const float q[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     const float spn = 1.0f;
     // NonLinearGetSampleDepthAndWeight (Octree.cpp:1403-1427)
     int d = S;
+    if (d < min_level) return defer();
     const float weight = 1.0f / sample_weight(L, q, d);
     float depth;
     if (weight >= spn) {
@@ -351,6 +364,7 @@ __global__ __launch_bounds__(kT) void k_splat_recs(const float* __restrict__ p, 
         float old_a = weight, new_a = weight;
         while (new_a < spn && d > 0) {
             --d;
+            if (d < min_level) return defer();
             old_a = new_a;
             new_a = 1.0f / sample_weight(L, q, d);
         }
@@ -386,6 +400,7 @@ __global__ __launch_bounds__(kT) void k_splat_recs(const float* __restrict__ p, 
         }
     }
     for (int j = 0; j < 54; ++j) vals[54 * i + j] = (int32_t)(54 * i + j);
+    if (!retry) todo[i] = 0;
 }
 
 __global__ __launch_bounds__(kT) void k_nl_sum(const double* __restrict__ v64,
@@ -612,10 +627,13 @@ struct Work {
         if ((rc = hist.ensure(sort_hist_len(m)))) return rc;
         return sort_pairs_u64(keys.p, vals.p, ktmp.p, vtmp.p, hist.p, m, bits, st);
     }
+    // records, sort scratch and run buffers for up to m records
     int reserve(int64_t m) {
         int rc;
         if ((rc = keys.ensure(m)) || (rc = ktmp.ensure(m)) || (rc = vals.ensure(m)) ||
-            (rc = vtmp.ensure(m)))
+            (rc = vtmp.ensure(m)) || (rc = hist.ensure(sort_hist_len(m))) ||
+            (rc = flags.ensure(m + 1)) || (rc = start.ensure(m + 1)) ||
+            (rc = stmp.ensure(scan_tmp_len(m + 1))))
             return rc;
         return CAGR_OK;
     }
@@ -685,8 +703,10 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
     const int64_t per_depth = 27 * n;
     const int early = std::min(3, S + 1);   // depths summed on the side stream
     SideStream side;
+    // (every buffer the later passes need is allocated here: a hipFree would
+    // wait for the side stream and undo the overlap)
     if ((rc = side.create(st)) || (rc = w.reserve(54 * n)) ||
-        (rc = allv.ensure((size_t)(S + 1) * per_depth)) || (rc = ukeys.ensure(per_depth)))
+        (rc = allv.ensure((size_t)(S + 1) * per_depth)) || (rc = ukeys.ensure(54 * n)))
         return rc;
     // seg table + sums of depths [d0, d1) into group g, on stream s
     auto sums = [&](int d0, int d1, int g, hipStream_t s_tab, hipStream_t s_sum) -> int {
@@ -729,7 +749,6 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
         if (d + 1 == early && (rc = sums(0, early, 0, st, side.s))) return rc;
     }
     if ((rc = sums(early, S + 1, 1, st, st))) return rc;
-    CAGR_HIP(side.join());
     for (int d = S + 1; d <= kMaxD; ++d) {
         L.uk[d] = nullptr;
         L.cw[d] = nullptr;
@@ -739,19 +758,36 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
     // sample depths and normal-length splats: 54 records per sample, sorted by
     // cell, serial sum per cell, cells with |sum| > 1e-6 kept
     DBuf<uint64_t> nb_keys;
+    DBuf<double> v64;
+    DBuf<int32_t> todo, n_todo, flag, ftmp, dcnt;
     int32_t nnb = 0;
     {
         const int64_t m = 54 * n;
-        DBuf<double> v64;
         if ((rc = v64.ensure(m))) return rc;
+        // first the samples whose depth search stays at depths >= `early`
+        // (their weights are ready); the rest once the side stream's
+        // shallow sums are joined.  Usually no sample climbs that far, and
+        // the long shallow chains then overlap everything below.
+        if ((rc = todo.ensure(n)) || (rc = n_todo.ensure(1))) return rc;
+        CAGR_HIP(hipMemsetAsync(n_todo.p, 0, sizeof(int32_t), st));
         k_splat_recs<<<nblk(n), kT, 0, st>>>(p, n, S, D, dmin, std::log(4.0), (float)(2 << D), L,
-                                              w.keys.p, w.vals.p, v64.p);
+                                              early, false, todo.p, n_todo.p, w.keys.p, w.vals.p,
+                                              v64.p);
         CAGR_LAUNCHED();
+        int32_t deferred = 0;
+        CAGR_HIP(hipMemcpyAsync(&deferred, n_todo.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CAGR_HIP(hipStreamSynchronize(st));
+        if (deferred) {
+            CAGR_HIP(side.join());
+            k_splat_recs<<<nblk(n), kT, 0, st>>>(p, n, S, D, dmin, std::log(4.0), (float)(2 << D),
+                                                  L, 0, true, todo.p, n_todo.p, w.keys.p,
+                                                  w.vals.p, v64.p);
+            CAGR_LAUNCHED();
+        }
         if ((rc = w.sort(m, kCellBits))) return rc;
         if ((rc = ukeys.ensure(m))) return rc;
         int32_t U = 0;
         if ((rc = w.runs(m, kNoCell, ukeys.p, &U))) return rc;
-        DBuf<int32_t> flag, ftmp;
         if ((rc = flag.ensure((size_t)U + 1)) || (rc = ftmp.ensure(scan_tmp_len((int64_t)U + 1))))
             return rc;
         k_nl_sum<<<nblk(64 * ((int64_t)U + 1)), kT, 0, st>>>(v64.p, w.vals.p, w.start.p, U,
@@ -763,18 +799,16 @@ int pgr_build(cagr_pgr* g, const float* pts, int64_t n, int D, int dmin, hipStre
         if ((rc = nb_keys.ensure(nnb))) return rc;
         k_compact<<<nblk(U), kT, 0, st>>>(ukeys.p, flag.p, U, nb_keys.p);
         CAGR_LAUNCHED();
-        CAGR_HIP(hipStreamSynchronize(st));  // before the block's buffers are freed
     }
 
     // cells with normals per depth (the keys are depth-major); the tree height
     // is the deepest of them
     int32_t nb_off[17] = {0};
     {
-        DBuf<int32_t> c;
-        if ((rc = c.ensure(17))) return rc;
-        k_depth_off<<<1, 64, 0, st>>>(nb_keys.p, nnb, c.p);
+        if ((rc = dcnt.ensure(17))) return rc;
+        k_depth_off<<<1, 64, 0, st>>>(nb_keys.p, nnb, dcnt.p);
         CAGR_LAUNCHED();
-        CAGR_HIP(hipMemcpyAsync(nb_off, c.p, sizeof(nb_off), hipMemcpyDeviceToHost, st));
+        CAGR_HIP(hipMemcpyAsync(nb_off, dcnt.p, sizeof(nb_off), hipMemcpyDeviceToHost, st));
         CAGR_HIP(hipStreamSynchronize(st));
     }
     int32_t cnt[16];
