@@ -68,7 +68,31 @@ struct View {
     const uint64_t* leaf_mk;       // leaves' left-aligned Morton keys (preorder = ascending)
     const uint8_t* leaf_mc;        // per leaf
     int64_t n_leaves;
+    // optional open-addressing tables (slot = key + 1, 0 empty; linear
+    // probing) answering the same lookups as the sorted arrays in a probe or two
+    const uint64_t* in_slot;       // internal node key -> position in in_keys
+    const int32_t* in_val;
+    uint64_t in_mask;
+    const uint64_t* lf_slot;       // leaf Morton key -> leaf position
+    const int32_t* lf_val;
+    uint64_t lf_mask;
 };
+
+CAGR_HD inline uint64_t hash_mix(uint64_t k) {      // splitmix64 finaliser
+    k ^= k >> 30;
+    k *= 0xbf58476d1ce4e5b9ull;
+    k ^= k >> 27;
+    k *= 0x94d049bb133111ebull;
+    return k ^ (k >> 31);
+}
+CAGR_HD inline int64_t hash_find(const uint64_t* slot, const int32_t* val, uint64_t mask,
+                                 uint64_t key) {
+    for (uint64_t h = hash_mix(key) & mask;; h = (h + 1) & mask) {
+        const uint64_t s = slot[h];
+        if (s == key + 1) return val[h];
+        if (s == 0) return -1;
+    }
+}
 
 struct RootInfo {
     Node node;
@@ -149,6 +173,7 @@ CAGR_HD inline bool is_internal(const View& t, int d, int64_t x, int64_t y, int6
     const int64_t lim = (int64_t)1 << d;
     if (x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) return false;
     const uint64_t k = node_key(d, x, y, z);
+    if (t.in_slot) return hash_find(t.in_slot, t.in_val, t.in_mask, k) >= 0;
     const int64_t i = lower_bound(t.in_keys, t.n_in, k);
     return i < t.n_in && t.in_keys[i] == k;
 }
@@ -188,11 +213,18 @@ CAGR_HD inline int mc_from_corners(const View& t, const Node& n);
 CAGR_HD inline int mc_index(const View& t, const Node& n) {
     if (t.in_mc) {
         const uint64_t k = node_key(n.d, n.o[0], n.o[1], n.o[2]);
-        const int64_t i = lower_bound(t.in_keys, t.n_in, k);
-        if (i < t.n_in && t.in_keys[i] == k) return t.in_mc[i];
         const uint64_t m = morton_left(n, t.md - 1);
-        const int64_t j = lower_bound(t.leaf_mk, t.n_leaves, m);
-        if (j < t.n_leaves && t.leaf_mk[j] == m) return t.leaf_mc[j];
+        if (t.in_slot) {
+            const int64_t i = hash_find(t.in_slot, t.in_val, t.in_mask, k);
+            if (i >= 0) return t.in_mc[i];
+            const int64_t j = hash_find(t.lf_slot, t.lf_val, t.lf_mask, m);
+            if (j >= 0) return t.leaf_mc[j];
+        } else {
+            const int64_t i = lower_bound(t.in_keys, t.n_in, k);
+            if (i < t.n_in && t.in_keys[i] == k) return t.in_mc[i];
+            const int64_t j = lower_bound(t.leaf_mk, t.n_leaves, m);
+            if (j < t.n_leaves && t.leaf_mk[j] == m) return t.leaf_mc[j];
+        }
     }
     return mc_from_corners(t, n);
 }

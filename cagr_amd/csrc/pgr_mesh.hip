@@ -126,6 +126,23 @@ __global__ __launch_bounds__(kT) void k_leaf_morton(View t, const int32_t* __res
     mc[r] = (uint8_t)mc_from_corners(t, n);
 }
 
+// open-addressing table of `keys` (slot = key + 1) -> position
+__global__ __launch_bounds__(kT) void k_hash_build(const uint64_t* __restrict__ keys, int64_t n,
+                                                   uint64_t mask,
+                                                   unsigned long long* __restrict__ slot,
+                                                   int32_t* __restrict__ val) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= n) return;
+    const unsigned long long k = keys[r] + 1;
+    for (uint64_t h = hash_mix(keys[r]) & mask;; h = (h + 1) & mask) {
+        const unsigned long long prev = atomicCAS(&slot[h], 0ull, k);
+        if (prev == 0ull || prev == k) {
+            val[h] = (int32_t)r;
+            return;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kT) void k_inner_mc(View t, int64_t n_in, uint8_t* __restrict__ mc) {
     const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (r >= n_in) return;
@@ -158,8 +175,9 @@ __global__ __launch_bounds__(kT) void k_mc_active(View t, const int32_t* __restr
         // the deepest existing ancestor-or-self of the neighbour cell
         while (c.d > 0 && !is_internal(t, parent(c))) c = parent(c);
         if (c.d == n.d) continue;                       // same depth: not coarser
-        const int64_t i = lower_bound(mk, nl, morton_left(c, H));
-        if (i < nl) active[i] = 1;
+        const int64_t i = t.lf_slot ? hash_find(t.lf_slot, t.lf_val, t.lf_mask, morton_left(c, H))
+                                    : lower_bound(mk, nl, morton_left(c, H));
+        if (i >= 0 && i < nl) active[i] = 1;
     }
 }
 
@@ -283,6 +301,33 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     t.leaf_mk = mk.p;
     t.leaf_mc = leaf_mc.p;
     t.n_leaves = nl;
+    // hash tables of the internal nodes and of the leaves' Morton keys
+    auto pow2 = [](int64_t n) {
+        uint64_t s = 1024;
+        while (s < 2 * (uint64_t)n) s <<= 1;
+        return s;
+    };
+    const uint64_t in_size = pow2(g->n_in), lf_size = pow2(nl);
+    Buf<unsigned long long> in_slot, lf_slot;
+    Buf<int32_t> in_val, lf_val;
+    if ((rc = in_slot.alloc(in_size)) || (rc = in_val.alloc(in_size)) ||
+        (rc = lf_slot.alloc(lf_size)) || (rc = lf_val.alloc(lf_size)))
+        return rc;
+    CAGR_HIP(hipMemsetAsync(in_slot.p, 0, in_size * sizeof(unsigned long long), st));
+    CAGR_HIP(hipMemsetAsync(lf_slot.p, 0, lf_size * sizeof(unsigned long long), st));
+    if (g->n_in) {
+        k_hash_build<<<nblk(g->n_in), kT, 0, st>>>(g->in_keys, g->n_in, in_size - 1, in_slot.p,
+                                                    in_val.p);
+        CAGR_LAUNCHED();
+    }
+    k_hash_build<<<nblk(nl), kT, 0, st>>>(mk.p, nl, lf_size - 1, lf_slot.p, lf_val.p);
+    CAGR_LAUNCHED();
+    t.in_slot = (const uint64_t*)in_slot.p;
+    t.in_val = in_val.p;
+    t.in_mask = in_size - 1;
+    t.lf_slot = (const uint64_t*)lf_slot.p;
+    t.lf_val = lf_val.p;
+    t.lf_mask = lf_size - 1;
 
     // vertices: root keys per (leaf, slot), the valid ones compacted and
     // sorted stably by key
