@@ -676,6 +676,15 @@ def config_pgr(args):
                            "triangles": int(mt.shape[0]),
                            "field": "FP32c -wn at the corners, its isovalue, smoothWidth w/2"}
             log(f"pgr mesh: {ms * 1e3:.1f} ms, {mv.shape[0]} vertices, {mt.shape[0]} triangles")
+    # the whole of main_GaussRecon_cuda.cu as one call (octree, treecode,
+    # -wn at corners and samples, isovalue, mesh, vertices back to the host)
+    from cagr_amd.pgr import gauss_recon
+    nml_t = torch.from_numpy(nml).to(dev)
+    gauss_recon(pt, nml_t, None, width=width, max_depth=D, min_depth=M, precision="fp32c")
+    gs, (gv, gt, _) = _sync_time(lambda: gauss_recon(pt, nml_t, None, width=width, max_depth=D,
+                                                     min_depth=M, precision="fp32c"))
+    out["gauss_recon_s"] = round(gs, 4)
+    log(f"pgr: gauss_recon end to end {gs * 1e3:.1f} ms ({len(gv)} vertices, {len(gt)} triangles)")
     if not args.no_cpu:
         kind = "reference" if oracle.pgr_ref_available() else "port"
         fn = oracle.pgr_ref_build if kind == "reference" else oracle.pgr_build
