@@ -209,3 +209,25 @@ def save_ply_mesh(path: str, vertices, triangles) -> None:
         f.write(header.encode("ascii"))
         f.write(v.tobytes())
         f.write(faces.tobytes())
+
+
+def reconstruct(path: str, out_path: Optional[str], *, max_depth: int = 10, min_depth: int = 1,
+                width: float = 0.01, device: str = "cuda", precision: Optional[str] = None):
+    """GaussRecon on an oriented point file, as ``main_GaussReconCUDA -i path -o
+    out_path -d max_depth -m min_depth -w width`` (main_GaussRecon_cuda.cu:
+    225-498): the samples and normals of ``path`` (x y z nx ny nz, read as
+    float32 like the reference's fscanf), the PGR octree, -wn at its corners,
+    the isovalue, marching cubes and the binary PLY at ``out_path`` (skipped
+    when None).  Returns (vertices [V,3] float32 in the input's coordinates,
+    triangles [T,3] int32, isovalue)."""
+    import torch
+
+    from .pgr import gauss_recon
+
+    raw, nrm = load_points(path)
+    if nrm is None:
+        raise ValueError(f"{path} holds no normals (GaussRecon reads x y z nx ny nz)")
+    pts = torch.from_numpy(np.ascontiguousarray(raw, dtype=np.float32)).to(device)
+    nml = torch.from_numpy(np.ascontiguousarray(nrm, dtype=np.float32)).to(device)
+    return gauss_recon(pts, nml, out_path, width=width, max_depth=max_depth,
+                       min_depth=min_depth, precision=precision)

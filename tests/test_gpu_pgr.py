@@ -163,3 +163,22 @@ def test_gauss_recon_end_to_end(tmp_path):
     # FP32c: the same surface up to the operator's tolerance
     v32, t32, _ = gauss_recon(pts, nml, None, max_depth=10, precision="fp32c")
     assert abs(len(v32) - nv) <= 0.01 * nv
+
+
+def test_reconstruct_from_an_oriented_point_file(tmp_path):
+    """pcio.reconstruct = main_GaussReconCUDA -i file -o mesh.ply: the same mesh
+    and PLY bytes as gauss_recon on the float32 samples and normals."""
+    from cagr_amd import pcio
+    from cagr_amd.pgr import gauss_recon
+    say("reconstruct from file")
+    pts = BUNNY["points"].astype(np.float32)
+    nml = BUNNY["normals"].astype(np.float32)
+    src = str(tmp_path / "bunny.xyz")
+    np.savetxt(src, np.hstack([pts, nml]), fmt="%.9g")
+    a = str(tmp_path / "a.ply")
+    b = str(tmp_path / "b.ply")
+    v1, t1, i1 = pcio.reconstruct(src, a, max_depth=8, precision="fp64")
+    v2, t2, i2 = gauss_recon(torch.from_numpy(pts).to(DEV), torch.from_numpy(nml).to(DEV), b,
+                             max_depth=8, precision="fp64")
+    assert i1 == i2 and np.array_equal(v1, v2) and np.array_equal(t1, t2)
+    assert open(a, "rb").read() == open(b, "rb").read()
