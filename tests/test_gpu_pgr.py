@@ -182,3 +182,19 @@ def test_reconstruct_from_an_oriented_point_file(tmp_path):
                              max_depth=8, precision="fp64")
     assert i1 == i2 and np.array_equal(v1, v2) and np.array_equal(t1, t2)
     assert open(a, "rb").read() == open(b, "rb").read()
+
+
+@pytest.mark.parametrize("name", ["line", "lattice", "outliers", "sparse"])
+def test_pgr_octree_degenerate_and_sparse_inputs(name):
+    """Collinear samples, exact cell-boundary ties, outliers and sparse
+    samples (whose depth search climbs to the shallow depths that the build
+    defers to its second pass) match the oracle exactly."""
+    rng = np.random.default_rng(71)
+    p = {"line": np.stack([np.linspace(-1, 1, 400)] * 3, 1),
+         "lattice": np.stack(np.meshgrid(*[np.arange(9) / 8.0] * 3), -1).reshape(-1, 3),
+         "outliers": np.concatenate([synth.sphere(3000, 72), rng.uniform(-1, 1, (150, 3))]),
+         "sparse": rng.uniform(-1, 1, (40, 3))}[name].astype(np.float32)
+    say("pgr degenerate", name)
+    for d, m in ((9, 1), (6, 0)):
+        o = oracle.pgr_build(p, d, m)
+        same(PGROctree(torch.from_numpy(p).to(DEV), d, m), o)
