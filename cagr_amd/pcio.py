@@ -212,14 +212,16 @@ def save_ply_mesh(path: str, vertices, triangles) -> None:
 
 
 def reconstruct(path: str, out_path: Optional[str], *, max_depth: int = 10, min_depth: int = 1,
-                width: float = 0.01, device: str = "cuda", precision: Optional[str] = None):
+                width: float = 0.01, device: str = "cuda", precision: Optional[str] = None,
+                op: str = "A3"):
     """GaussRecon on an oriented point file, as ``main_GaussReconCUDA -i path -o
     out_path -d max_depth -m min_depth -w width`` (main_GaussRecon_cuda.cu:
     225-498): the samples and normals of ``path`` (x y z nx ny nz, read as
     float32 like the reference's fscanf), the PGR octree, -wn at its corners,
     the isovalue, marching cubes and the binary PLY at ``out_path`` (skipped
-    when None).  Returns (vertices [V,3] float32 in the input's coordinates,
-    triangles [T,3] int32, isovalue)."""
+    when None); ``op="A_cpu"`` runs main_GaussReconCPU's field instead.
+    Returns (vertices [V,3] float32 in the input's coordinates, triangles
+    [T,3] int32, isovalue)."""
     import torch
 
     from .pgr import gauss_recon
@@ -230,4 +232,4 @@ def reconstruct(path: str, out_path: Optional[str], *, max_depth: int = 10, min_
     pts = torch.from_numpy(np.ascontiguousarray(raw, dtype=np.float32)).to(device)
     nml = torch.from_numpy(np.ascontiguousarray(nrm, dtype=np.float32)).to(device)
     return gauss_recon(pts, nml, out_path, width=width, max_depth=max_depth,
-                       min_depth=min_depth, precision=precision)
+                       min_depth=min_depth, precision=precision, op=op)

@@ -126,22 +126,26 @@ def output_transform(verts01, box, scale) -> np.ndarray:
 def gauss_recon_values(points: torch.Tensor, normals: torch.Tensor, *, width: float = 0.01,
                        max_depth: int = 10, min_depth: int = 1, area: float = 1e-5,
                        leaf_size: int = 1, theta: float = 0.5,
-                       precision: Optional[str] = None):
-    """main_GaussRecon_cuda.cu:255-486 before marching cubes.
+                       precision: Optional[str] = None, op: str = "A3"):
+    """main_GaussRecon_cuda.cu:255-486 before marching cubes (``op="A3"``);
+    ``op="A_cpu"`` is main_GaussRecon_cpu.cpp:303-359 instead, whose field is
+    the CPU-sign plain operator multiply_by_A with the continuous kernel.
 
     Returns ``(octree, values [G], isovalue, smooth_width)``: -wn at every
     corner of the PGR octree in ``gridDataVector`` order, -wn at the samples
     (element N/2 after nth_element) and width / 2.  Defaults are the
     reference's (-d 10, -m 1, -w 0.01, leaf size 1, θ = 0.5).
     """
+    if op not in ("A3", "A_cpu"):
+        raise ValueError("op must be 'A3' (GaussRecon CUDA) or 'A_cpu' (GaussRecon CPU)")
     oct_ = PGROctree(points, max_depth, min_depth)
     pts = oct_.sample_points()
     mu = sample_attributes(normals.to(device=pts.device, dtype=pts.dtype), area)
     wt = WindingNumberTreecode(pts, leaf_size=leaf_size, theta=theta, precision=precision)
-    wn_q = wt.query(oct_.query_points(), mu, width, precision=precision)[:, 0]
+    wn_q = wt.query(oct_.query_points(), mu, width, op=op, precision=precision)[:, 0]
     n = pts.shape[0]
     widths = torch.full((n,), width, dtype=pts.dtype, device=pts.device)
-    wn_s = wt.apply("A3", mu, widths, continuous=True, precision=precision)[:, 0]
+    wn_s = wt.apply(op, mu, widths, continuous=True, precision=precision)[:, 0]
     kth = torch.kthvalue(wn_s.double().cpu(), n // 2 + 1).values
     return oct_, -wn_q, float(-kth), width / 2
 
@@ -149,17 +153,18 @@ def gauss_recon_values(points: torch.Tensor, normals: torch.Tensor, *, width: fl
 def gauss_recon(points: torch.Tensor, normals: torch.Tensor, out_path: Optional[str] = None, *,
                 width: float = 0.01, max_depth: int = 10, min_depth: int = 1,
                 area: float = 1e-5, leaf_size: int = 1, theta: float = 0.5,
-                precision: Optional[str] = None):
-    """The whole of main_GaussRecon_cuda.cu (:255-498): PGR octree, -wn at its
-    corners and the isovalue, marching cubes, and the binary PLY of
-    writePolygon2 when ``out_path`` is given.  Returns ``(vertices [V,3] in
+                precision: Optional[str] = None, op: str = "A3"):
+    """The whole of main_GaussRecon_cuda.cu (:255-498; ``op="A_cpu"``:
+    main_GaussRecon_cpu.cpp): PGR octree, -wn at its corners and the
+    isovalue, marching cubes, and the binary PLY of writePolygon2 when
+    ``out_path`` is given.  Returns ``(vertices [V,3] in
     the input's coordinates (numpy float32), triangles [T,3] (numpy int32),
     isovalue)``."""
     from .pcio import save_ply_mesh
     oct_, vals, iso, sw = gauss_recon_values(points, normals, width=width, max_depth=max_depth,
                                             min_depth=min_depth, area=area,
                                             leaf_size=leaf_size, theta=theta,
-                                            precision=precision)
+                                            precision=precision, op=op)
     verts, tris = oct_.mesh(vals.float(), iso, sw)
     out_v = oct_.output_vertices(verts)
     out_t = tris.cpu().numpy()

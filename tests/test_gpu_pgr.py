@@ -16,7 +16,7 @@ import torch
 
 import oracle
 from cagr_amd import _lib, synth
-from cagr_amd.pgr import PGROctree, gauss_recon_values
+from cagr_amd.pgr import PGROctree, gauss_recon, gauss_recon_values
 from cagr_amd.query import sample_attributes
 
 pytestmark = pytest.mark.gpu
@@ -198,3 +198,36 @@ def test_pgr_octree_degenerate_and_sparse_inputs(name):
     for d, m in ((9, 1), (6, 0)):
         o = oracle.pgr_build(p, d, m)
         same(PGROctree(torch.from_numpy(p).to(DEV), d, m), o)
+
+
+
+def test_gauss_recon_cpu_driver_field():
+    """main_GaussRecon_cpu.cpp:303-359: the field is the CPU-sign plain
+    operator multiply_by_A (continuous kernel) at the corners and samples."""
+    pts = BUNNY["points"].astype(np.float32)
+    nml = BUNNY["normals"]
+    say("gauss recon cpu-driver field")
+    o = oracle.pgr_build(pts, 8, 1)
+    p64 = (2 * o.samples - 1).astype(np.float64)
+    q64 = (2 * o.corners - 1).astype(np.float64)
+    mu = sample_attributes(torch.from_numpy(nml).to(DEV).float()).double().cpu().numpy()
+    ot = oracle.build_tree(p64, 15, 1)
+    ref_q = oracle.apply("A_cpu", ot, p64, mu, 0.01, queries=q64, continuous=True)[:, 0]
+    ref_s = oracle.apply("A_cpu", ot, p64, mu, 0.01, continuous=True)[:, 0]
+    ref_iso = -np.partition(ref_s, len(ref_s) // 2)[len(ref_s) // 2]
+    t_pts = torch.from_numpy(pts).to(DEV)
+    t_nml = torch.from_numpy(nml).to(DEV)
+    for prec, tol in (("fp64", 1e-12), ("fp32c", 2e-6)):
+        oct_, vals, iso, sw = gauss_recon_values(t_pts, t_nml, max_depth=8, precision=prec,
+                                                 op="A_cpu")
+        v = vals.double().cpu().numpy()
+        scale = np.abs(ref_q).max()
+        assert np.abs(v + ref_q).max() <= tol * scale + 1e-7 * scale, prec
+        assert abs(iso - ref_iso) <= 1e-6 * np.abs(ref_s).max(), prec
+    # the whole driver: a closed mesh from the CPU-driver field
+    verts, tris, iso = gauss_recon(t_pts, t_nml, None, max_depth=8, precision="fp64", op="A_cpu")
+    assert abs(iso - ref_iso) <= 1e-6 * np.abs(ref_s).max()
+    assert len(verts) > 100 and len(tris) > 100
+    assert tris.min() >= 0 and tris.max() < len(verts)
+    with pytest.raises(ValueError):
+        gauss_recon_values(t_pts, t_nml, max_depth=8, op="G")
