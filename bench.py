@@ -25,6 +25,10 @@ residual replacement (its adjoint through an adjoint plan, whose build is
 timed inside the solve; also reported without the plan); all start from
 mu = 0 on the tree built above.
 
+Seeds (seed_sweep): the same workload on seeds 1..--seeds-1 (new clouds and
+attributes; same setup, steps and solves), with the medians over all seeds.
+``value`` and the roofline stay those of the seed-0 run (--seed).
+
 Prints one JSON line.
 
 The other configurations of the metric's table run on request, each printing
@@ -77,6 +81,9 @@ def parse():
     ap.add_argument("--solve-tol", type=float, default=1e-3,
                     help="FP64-verified relative residual of the solve comparison")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--seeds", type=int, default=5,
+                    help="1M config: also time seeds 1..SEEDS-1 (pairs/s, FP64 ratio, solves) "
+                         "and report the medians over seeds 0..SEEDS-1")
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "pgr"],
                     help="c3 (default): the headline workload; others: see the module docstring")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -288,6 +295,31 @@ def main():
         log(f"solve: speedup {solve['speedup']:.2f}x, ||mu32 - mu64|| / ||mu64|| = {dmu:.2e}")
         del mu64, mu32, op, opp
 
+    # ---------------------------------------------------------------- other seeds
+    sweep = None
+    if args.seeds > 1:
+        sweep = {"seeds": [args.seed], "fp32c_pairs_per_s": [round(value, 4)],
+                 "fp64_pairs_per_s": [round(fp64["value"], 4)] if fp64 else None,
+                 "solve_speedup": [solve["speedup"]] if solve else None}
+        del wt
+        torch.cuda.empty_cache()
+        for sd in range(args.seed + 1, args.seed + args.seeds):
+            r = seed_run(args, sd, fp64 is not None, solve is not None)
+            sweep["seeds"].append(sd)
+            sweep["fp32c_pairs_per_s"].append(r["fp32c"])
+            if fp64:
+                sweep["fp64_pairs_per_s"].append(r["fp64"])
+            if solve:
+                sweep["solve_speedup"].append(r["solve_speedup"])
+        sweep["median_fp32c_pairs_per_s"] = float(np.median(sweep["fp32c_pairs_per_s"]))
+        if fp64:
+            sweep["median_speedup_vs_fp64"] = round(float(np.median(
+                np.array(sweep["fp32c_pairs_per_s"]) / np.array(sweep["fp64_pairs_per_s"]))), 3)
+        if solve:
+            sweep["median_solve_speedup"] = float(np.median(sweep["solve_speedup"]))
+        log(f"seeds {sweep['seeds']}: fp32c {sweep['fp32c_pairs_per_s']} pairs/s, median "
+            f"{sweep['median_fp32c_pairs_per_s']:.2f}")
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -337,9 +369,49 @@ def main():
         line["fp64"] = fp64
     if solve:
         line["solve"] = solve
+    if sweep:
+        line["seed_sweep"] = sweep
     if cpu:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
+
+
+def seed_run(args, seed, with_fp64, with_solve):
+    """The 1M workload on another seed: same setup as the main run (tree
+    build, waves ordered longest first), then timed FP32c and FP64 pairs and
+    the two tolerance solves."""
+    import torch
+    from cagr_amd import WindingNumberTreecode, synth
+    from cagr_amd.solve import cgnr_solve, cgnr_solve_mixed
+    dev = torch.device("cuda:0")
+    n = args.points
+    pts = synth.normalize(synth.ellipsoids(n, seed)).astype(np.float32)
+    mu = torch.from_numpy(synth.random_normals(n, seed + 1).astype(np.float32)).to(dev)
+    w = torch.full((n,), args.width, dtype=torch.float32, device=dev)
+    wt = WindingNumberTreecode(torch.from_numpy(pts).to(dev), leaf_size=args.leaf,
+                               theta=args.theta, precision="fp32c")
+    wt.balance("A2", mu)
+    out = {}
+    for prec, steps, warm in (("fp32c", args.steps, args.warmup),
+                              ("fp64", args.fp64_steps or max(2, args.steps // 4), 1)):
+        if prec == "fp64" and not with_fp64:
+            continue
+        for _ in range(warm):
+            wt.forward_A2T_sum(-wt.forward_A2(mu, w, precision=prec), w, precision=prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            wt.forward_A2T_sum(-wt.forward_A2(mu, w, precision=prec), w, precision=prec)
+        torch.cuda.synchronize()
+        out[prec] = round(steps / (time.perf_counter() - t), 4)
+    if with_solve:
+        _, i64 = cgnr_solve(wt, args.width, tol=args.solve_tol)
+        _, i32 = cgnr_solve_mixed(wt, args.width, tol=args.solve_tol)
+        out["solve_speedup"] = round(i64["seconds"] / i32["seconds"], 3)
+    log(f"seed {seed}: {out}")
+    del wt
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(pts, mu_np, s3, args, r32, a32):
