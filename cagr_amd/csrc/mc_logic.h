@@ -22,6 +22,7 @@
 //     construction, and ReconOctNode::__edgeNeighbor returns either that
 //     cell, NULL or a childless coarser node.
 #pragma once
+#include <cstddef>
 #include <cstdint>
 
 #include "mc_tables.h"
@@ -354,27 +355,95 @@ CAGR_HD inline bool root_pair(const View& t, const RootInfo& ri, RootInfo& pair)
 }
 
 // ------------------------------------------------------------ per-leaf polygons
-// workspace of one leaf (caller-provided; device code points it at global memory)
+// workspace of one leaf.  Its arrays are lane-strided views into
+// caller-provided memory: element i of a lane's array lies kStride elements
+// after its element i - 1.  The kernels interleave the workspaces of one
+// wave (kStride = 64, lane l at offset l), so that the lanes of a wave that
+// touch the same index read one contiguous run instead of 64 cache lines;
+// the host replay uses kStride = 1.  The counters are plain members.
+#ifndef CAGR_MC_STRIDE
+#define CAGR_MC_STRIDE 64
+#endif
+constexpr int kStride = CAGR_MC_STRIDE;
+
+template <class T>
+struct Lanes {
+    T* p;
+    CAGR_HD T& operator[](int i) const { return p[(int64_t)i * kStride]; }
+};
+template <class T, int N>
+struct Vec {   // a small fixed array kept together as one element
+    T v[N];
+    CAGR_HD T& operator[](int i) { return v[i]; }
+    CAGR_HD const T& operator[](int i) const { return v[i]; }
+};
+
+constexpr size_t align8(size_t x) { return (x + 7) & ~size_t(7); }
+
+// byte offsets of the arrays in one lane's share (multiples of 8), and its size
+struct WorkLayout {
+    static constexpr size_t ea = 0;
+    static constexpr size_t eb = align8(ea + sizeof(int64_t) * kMaxEdges);
+    static constexpr size_t vkey = align8(eb + sizeof(int64_t) * kMaxEdges);
+    static constexpr size_t vcount = align8(vkey + sizeof(int64_t) * kMaxVerts);
+    static constexpr size_t vinfo = align8(vcount + sizeof(int) * kMaxVerts);
+    static constexpr size_t loop = align8(vinfo + sizeof(RootInfo) * kMaxVerts);
+    static constexpr size_t back = align8(loop + sizeof(int64_t) * kMaxEdges);
+    static constexpr size_t vid = align8(back + sizeof(int64_t) * kMaxEdges);
+    static constexpr size_t tp = align8(vid + sizeof(int) * kMaxEdges);
+    static constexpr size_t te_p = align8(tp + sizeof(Vec<float, 3>) * kMaxEdges);
+    static constexpr size_t te_t = align8(te_p + sizeof(Vec<int, 2>) * 3 * kMaxEdges);
+    static constexpr size_t tt_e = align8(te_t + sizeof(Vec<int, 2>) * 3 * kMaxEdges);
+    static constexpr size_t own_state = align8(tt_e + sizeof(Vec<int, 3>) * kMaxEdges);
+    static constexpr size_t own = align8(own_state + sizeof(int8_t) * 12);
+    static constexpr size_t out = align8(own + sizeof(RootInfo) * 12);
+    static constexpr size_t bytes = align8(out + sizeof(int32_t) * 3 * kMaxEdges);
+};
+
+template <class T>
+CAGR_HD inline Lanes<T> lanes_at(unsigned char* blk, size_t off, int lane) {
+    return Lanes<T>{reinterpret_cast<T*>(blk + off * kStride) + lane};
+}
+
 struct Work {
-    int64_t ea[kMaxEdges], eb[kMaxEdges];     // iso edges (first, second) by root key
-    int ne;
-    int64_t vkey[kMaxVerts];                   // vertexCount of the reference
-    int vcount[kMaxVerts];
-    RootInfo vinfo[kMaxVerts];
-    int nv;
-    int64_t loop[kMaxEdges];                   // one loop's vertex keys
-    int64_t back[kMaxEdges];                   // its back chain while it is built
-    int vid[kMaxEdges];                        // its vertex numbers
+    Lanes<int64_t> ea, eb;                     // iso edges (first, second) by root key
+    int ne = 0;
+    Lanes<int64_t> vkey;                       // vertexCount of the reference
+    Lanes<int> vcount;
+    Lanes<RootInfo> vinfo;
+    int nv = 0;
+    Lanes<int64_t> loop;                       // one loop's vertex keys
+    Lanes<int64_t> back;                       // its back chain while it is built
+    Lanes<int> vid;                            // its vertex numbers
     // triangulation (Geometry.cpp:74-220)
-    float tp[kMaxEdges][3];
-    int te_p[3 * kMaxEdges][2], te_t[3 * kMaxEdges][2];
-    int tt_e[kMaxEdges][3];
-    int nte, ntt;
-    bool overflow;
+    Lanes<Vec<float, 3>> tp;
+    Lanes<Vec<int, 2>> te_p, te_t;
+    Lanes<Vec<int, 3>> tt_e;
+    int nte = 0, ntt = 0;
+    bool overflow = false;
     // the leaf's own 12 edge roots, found on first use (0 unknown, 1 found, 2 none)
-    int8_t own_state[12];
-    RootInfo own[12];
-    int32_t out[3 * kMaxEdges];                // the leaf's triangles, before they are placed
+    Lanes<int8_t> own_state;
+    Lanes<RootInfo> own;
+    Lanes<int32_t> out;                        // the leaf's triangles, before they are placed
+
+    // lane `lane` of the kStride interleaved workspaces at `blk`
+    // (WorkLayout::bytes * kStride bytes, 8-byte aligned)
+    CAGR_HD Work(unsigned char* blk, int lane)
+        : ea(lanes_at<int64_t>(blk, WorkLayout::ea, lane)),
+          eb(lanes_at<int64_t>(blk, WorkLayout::eb, lane)),
+          vkey(lanes_at<int64_t>(blk, WorkLayout::vkey, lane)),
+          vcount(lanes_at<int>(blk, WorkLayout::vcount, lane)),
+          vinfo(lanes_at<RootInfo>(blk, WorkLayout::vinfo, lane)),
+          loop(lanes_at<int64_t>(blk, WorkLayout::loop, lane)),
+          back(lanes_at<int64_t>(blk, WorkLayout::back, lane)),
+          vid(lanes_at<int>(blk, WorkLayout::vid, lane)),
+          tp(lanes_at<Vec<float, 3>>(blk, WorkLayout::tp, lane)),
+          te_p(lanes_at<Vec<int, 2>>(blk, WorkLayout::te_p, lane)),
+          te_t(lanes_at<Vec<int, 2>>(blk, WorkLayout::te_t, lane)),
+          tt_e(lanes_at<Vec<int, 3>>(blk, WorkLayout::tt_e, lane)),
+          own_state(lanes_at<int8_t>(blk, WorkLayout::own_state, lane)),
+          own(lanes_at<RootInfo>(blk, WorkLayout::own, lane)),
+          out(lanes_at<int32_t>(blk, WorkLayout::out, lane)) {}
 };
 
 CAGR_HD inline int vfind(const Work& w, int64_t key) {
@@ -517,8 +586,8 @@ CAGR_HD inline int next_loop(Work& w) {
     int64_t front_idx = e0b, back_idx = e0a;
     // front / back chains kept in the loop buffer from both ends
     int nf = 0, nb = 0;
-    int64_t* fr = w.loop;                          // front[0..nf) firsts
-    int64_t* back = w.back;
+    const Lanes<int64_t> fr = w.loop;              // front[0..nf) firsts
+    const Lanes<int64_t> back = w.back;
     for (int j = w.ne - 1; j >= 0; --j) {
         if (w.ea[j] == front_idx || w.eb[j] == front_idx) {
             int64_t first, second;
@@ -667,7 +736,7 @@ CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VInd
     int count = 0;
     int len;
     while ((len = next_loop(w)) > 0) {
-        int* vid = w.vid;
+        const Lanes<int> vid = w.vid;
         int nv = 0;
         for (int j = 0; j < len; ++j) {
             const int v = vindex(w.loop[j]);
@@ -675,7 +744,7 @@ CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VInd
         }
         if (nv > 3) {
             w.nte = w.ntt = 0;
-            for (int j = 0; j < nv; ++j) vindex.position(vid[j], w.tp[j]);
+            for (int j = 0; j < nv; ++j) vindex.position(vid[j], w.tp[j].v);
             for (int j = 1; j < nv - 1; ++j) tri_add(w, 0, j, j + 1);
             for (;;) {
                 int e;

@@ -205,10 +205,11 @@ struct DevVIndex {
 // them in its workspace, reserves room in `tmp` with one atomic add and
 // copies them there; counts and reservations are kept per leaf so that
 // k_mc_place can move every leaf's run to its place in preorder.  Threads
-// persist over the leaves (grid stride), one workspace each.
+// persist over the leaves (grid stride), one workspace each; a block is one
+// wave, whose 64 workspaces are interleaved lane by lane in `ws`.
 __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restrict__ leaves,
                                                 const int32_t* __restrict__ ids, int64_t na,
-                                                DevVIndex vi, Work* __restrict__ ws,
+                                                DevVIndex vi, unsigned char* __restrict__ ws,
                                                 int32_t* __restrict__ counts,
                                                 int64_t* __restrict__ base,
                                                 unsigned long long* __restrict__ used,
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(64) void k_mc_tris(View t, const int32_t* __restric
                                                 int32_t* __restrict__ err) {
     const int64_t tid = (int64_t)blockIdx.x * 64 + threadIdx.x;
     const int64_t nthreads = (int64_t)gridDim.x * 64;
-    Work& w = ws[tid];
+    static_assert(kStride == 64, "one wave's workspaces are interleaved");
+    Work w(ws + (size_t)blockIdx.x * 64 * WorkLayout::bytes, threadIdx.x);
     for (int64_t a = tid; a < na; a += nthreads) {
         int k = 0;
         const int c = leaf_triangles(t, leaf_node(leaves, ids[a]), w, vi,
@@ -397,11 +399,11 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     // one workspace per thread; enough threads to keep every CU busy
     const int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(na, 256 * 64 * 12));
     const int64_t nb = (nthreads + 63) / 64;
-    Buf<Work> ws;
+    Buf<unsigned char> ws;
     Buf<int32_t> cnt, err;
     Buf<int64_t> base, offs;
     Buf<unsigned long long> used;
-    if ((rc = ws.alloc(nb * 64)) || (rc = cnt.alloc(na + 1)) || (rc = err.alloc(1)) ||
+    if ((rc = ws.alloc((size_t)nb * 64 * WorkLayout::bytes)) || (rc = cnt.alloc(na + 1)) || (rc = err.alloc(1)) ||
         (rc = base.alloc(na)) || (rc = offs.alloc(na + 1)) || (rc = used.alloc(1)))
         return rc;
     const DevVIndex vi{vkeys.p, vidx.p, nv, g->mesh_v};

@@ -4,6 +4,7 @@
 // preorder, edge slot), then triangles leaf by leaf.  The test compares the
 // result with the reference's own mesh.
 #define CAGR_HD
+#define CAGR_MC_STRIDE 1
 #include <cmath>
 using std::sqrt;
 #include "../../cagr_amd/csrc/mc_logic.h"
@@ -110,10 +111,11 @@ extern "C" int hl_mesh(const int32_t* leaves, int64_t nl, int md, const float* v
             for (int k = 0; k < 3; ++k) p[k] = (*pts)[3 * v + k];
         }
     } vi{&vid, &g_pts};
-    auto w = std::make_unique<Work>();
+    std::vector<uint64_t> wbuf(WorkLayout::bytes / 8 + 1);
+    Work w(reinterpret_cast<unsigned char*>(wbuf.data()), 0);
     for (int64_t r = 0; r < nl; ++r) {
         Node n{leaves[4 * r], {leaves[4 * r + 1], leaves[4 * r + 2], leaves[4 * r + 3]}};
-        const int c = leaf_triangles(t, n, *w, vi, [&](int a, int b, int cc) {
+        const int c = leaf_triangles(t, n, w, vi, [&](int a, int b, int cc) {
             g_tri.push_back(a);
             g_tri.push_back(b);
             g_tri.push_back(cc);
