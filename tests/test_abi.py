@@ -71,3 +71,16 @@ def test_no_cpu_fallback():
         WindingNumberTreecode(torch.zeros(10, 3))
     with pytest.raises(ValueError):
         PGROctree(torch.zeros(10, 3))
+
+
+def test_gauss_recon_op_is_checked_first():
+    """An operator other than the two GaussRecon drivers' fields (A3 of the
+    CUDA driver, A_cpu of the CPU driver) is refused before any device work."""
+    import torch
+    from cagr_amd.pgr import gauss_recon, gauss_recon_values
+    p = torch.zeros(10, 3)
+    for bad in ("G", "A", "A2"):
+        with pytest.raises(ValueError, match="op must be"):
+            gauss_recon_values(p, p, op=bad)
+        with pytest.raises(ValueError, match="op must be"):
+            gauss_recon(p, p, None, op=bad)
