@@ -24,6 +24,25 @@ int prof_mark(cagr_tree* t, int kind, hipStream_t st) {
     return CAGR_OK;
 }
 
+int call_begin(cagr_tree* t, hipStream_t st) {
+    CAGR_REQUIRE(t->done_ev != nullptr, CAGR_ERR_STATE, "tree handle was not built");
+    if (!t->ordered || t->last_stream == st) return CAGR_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    CAGR_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return CAGR_OK;   // the graph's caller orders it
+    CAGR_HIP(hipStreamWaitEvent(st, t->done_ev, 0));
+    return CAGR_OK;
+}
+
+void call_end(cagr_tree* t, hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!t->done_ev || hipStreamIsCapturing(st, &cs) != hipSuccess ||
+        cs != hipStreamCaptureStatusNone)
+        return;   // a captured call is ordered by the graph's caller
+    t->ordered = hipEventRecord(t->done_ev, st) == hipSuccess;
+    t->last_stream = st;
+}
+
 // Morton order of arbitrary query points (implemented in tree_build.hip)
 int morton_order(const void* pts, int dtype, int64_t n, int32_t* order, void* scratch,
                  hipStream_t st);
@@ -84,6 +103,8 @@ int apply_impl(cagr_tree* t, int op, int precision, const void* attrs, int dtype
     if (t->n == 0) return CAGR_OK;
     CAGR_REQUIRE(attrs && widths && out, CAGR_ERR_ARG, "null attrs / widths / out");
     hipStream_t st = (hipStream_t)stream;
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
     if ((rc = prof_mark(t, 1, st))) return rc;
     rc = run_upward_for(t, op, precision, oi, attrs, dtype, weights, an, st);
     if (rc) return rc;
@@ -302,6 +323,7 @@ int cagr_tree_destroy(cagr_tree* t) {
     (void)hipFree(t->item_counter);
     (void)hipFree(t->adj_node);
     (void)hipFree(t->adj_pt);
+    if (t->done_ev) (void)hipEventDestroy(t->done_ev);
     delete t;
     return CAGR_OK;
 }
@@ -362,8 +384,10 @@ int cagr_upward(cagr_tree* t, const void* attrs, int dtype, int attr_dim, const 
     CAGR_REQUIRE(attrs && node_attrs && node_reppoints && node_weights, CAGR_ERR_ARG,
                  "null argument");
     hipStream_t st = (hipStream_t)stream;
-    const int rc = upward(t, attr_dim == 3 ? 0 : 1, CAGR_PREC_FP64, 1, 0, attrs, dtype, weights,
-                          0.0, st);
+    int rc = call_begin(t, st);
+    if (rc) return rc;
+    CallEnd end_{t, st};
+    rc = upward(t, attr_dim == 3 ? 0 : 1, CAGR_PREC_FP64, 1, 0, attrs, dtype, weights, 0.0, st);
     if (rc) return rc;
     CAGR_HIP(hipMemcpyAsync(node_attrs, t->m64, t->m * attr_dim * sizeof(double),
                             hipMemcpyDeviceToDevice, st));
@@ -397,6 +421,8 @@ int cagr_apply_adjoint(cagr_tree* t, int op, int precision, const void* g, int d
     if (t->n == 0) return CAGR_OK;
     CAGR_REQUIRE(g && weights && widths && out, CAGR_ERR_ARG, "null g / weights / widths / out");
     hipStream_t st = (hipStream_t)stream;
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
     // node centres from the fixed weights (the moments are not needed)
     if ((rc = prof_mark(t, 1, st))) return rc;
     rc = upward(t, 0, precision, 1, 0, nullptr, dtype, weights, oi.convects ? an : 0.0, st);
@@ -414,7 +440,11 @@ int cagr_adjoint_plan(cagr_tree* t, const void* weights, int dtype, void* stream
     CAGR_REQUIRE(t->num_clouds == 1, CAGR_ERR_ARG, "the adjoint plan needs a single-cloud tree");
     if (t->n == 0) return CAGR_OK;
     CAGR_REQUIRE(weights, CAGR_ERR_ARG, "null weights");
-    const int rc = adjoint_plan(t, weights, dtype, (hipStream_t)stream);
+    const hipStream_t st = (hipStream_t)stream;
+    int rc = call_begin(t, st);
+    if (rc) return rc;
+    CallEnd end_{t, st};
+    rc = adjoint_plan(t, weights, dtype, st);
     if (rc) adjoint_plan_free(t);
     return rc;
 }
@@ -432,6 +462,8 @@ int cagr_apply_adjoint_planned(cagr_tree* t, int op, int precision, const void* 
     CAGR_REQUIRE(t->plan_slots >= 0, CAGR_ERR_STATE, "no adjoint plan: call cagr_adjoint_plan");
     CAGR_REQUIRE(g && widths && out, CAGR_ERR_ARG, "null g / widths / out");
     hipStream_t st = (hipStream_t)stream;
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
     // the FP32 records of the plan's centres are kept after the first call
     // (per convection factor); FP64 recomputes them
     const double an_rec = oi.convects ? an : 0.0;
@@ -463,9 +495,27 @@ int cagr_apply_adjoint_planned(cagr_tree* t, int op, int precision, const void* 
     return prof_mark(t, 0, st);
 }
 
-int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+}  // extern "C"
+
+namespace cagr {
+namespace {
+
+__global__ void k_zero_words(uint32_t* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// query workspace: the Morton order [nq] int32, then the sort's scratch
+size_t query_ws_bytes(int64_t nq) { return nq * sizeof(int32_t) + morton_order_scratch(nq); }
+
+// the shared part of cagr_apply_queries and its no-allocation variant; `ws`
+// returns the query workspace (query_ws_bytes(nq) bytes) once the arguments
+// have been checked, or fails
+template <class Ws>
+int apply_queries_impl(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
                        const void* queries, int qdtype, int64_t nq, const void* widths,
-                       double width, int continuous, double an, void* out, void* stream) {
+                       double width, int continuous, double an, void* out, void* stream,
+                       Ws ws) {
     OpInfo oi;
     int rc = check_common(t, op, precision, dtype, &oi);
     if (rc) return rc;
@@ -474,22 +524,19 @@ int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, i
     if (nq == 0) return CAGR_OK;
     CAGR_REQUIRE(queries && out, CAGR_ERR_ARG, "null queries / out");
     hipStream_t st = (hipStream_t)stream;
-    if (t->n == 0) {   // no sources: every output is zero
-        CAGR_HIP(hipMemsetAsync(out, 0, nq * oi.out_dim * (dtype == CAGR_F32 ? 4 : 8), st));
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
+    if (t->n == 0) {   // no sources: every output is zero (a fill kernel, no memset node)
+        const int64_t words = nq * oi.out_dim * (dtype == CAGR_F32 ? 1 : 2);
+        k_zero_words<<<(words + 255) / 256, 256, 0, st>>>((uint32_t*)out, words);
+        CAGR_LAUNCHED();
         return CAGR_OK;
     }
     CAGR_REQUIRE(attrs, CAGR_ERR_ARG, "null attrs");
-    const size_t need = nq * sizeof(int32_t) + morton_order_scratch(nq);
-    if (need > t->ws_queries) {   // grow-only query workspace
-        CAGR_HIP(hipStreamSynchronize(st));
-        (void)hipFree(t->qws);
-        t->qws = nullptr;
-        t->ws_queries = 0;
-        CAGR_HIP(hipMalloc(&t->qws, need));
-        t->ws_queries = need;
-    }
-    int32_t* order = (int32_t*)t->qws;
-    rc = morton_order(queries, qdtype, nq, order, (char*)t->qws + nq * sizeof(int32_t), st);
+    char* w = nullptr;
+    if ((rc = ws(st, &w))) return rc;
+    int32_t* order = (int32_t*)w;
+    rc = morton_order(queries, qdtype, nq, order, w + nq * sizeof(int32_t), st);
     if (rc) return rc;
     rc = run_upward_for(t, op, precision, oi, attrs, dtype, nullptr, an, st);
     if (rc) return rc;
@@ -503,6 +550,31 @@ int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, i
     q.wdtype = dtype;
     q.width = width;
     return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
+}
+
+}  // namespace
+}  // namespace cagr
+
+extern "C" {
+
+int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+                       const void* queries, int qdtype, int64_t nq, const void* widths,
+                       double width, int continuous, double an, void* out, void* stream) {
+    auto grow = [t, nq](hipStream_t st, char** w) -> int {
+        const size_t need = query_ws_bytes(nq);
+        if (need > t->ws_queries) {   // grow-only query workspace
+            CAGR_HIP(hipStreamSynchronize(st));
+            (void)hipFree(t->qws);
+            t->qws = nullptr;
+            t->ws_queries = 0;
+            CAGR_HIP(hipMalloc(&t->qws, need));
+            t->ws_queries = need;
+        }
+        *w = (char*)t->qws;
+        return CAGR_OK;
+    };
+    return apply_queries_impl(t, op, precision, attrs, dtype, queries, qdtype, nq, widths, width,
+                              continuous, an, out, stream, grow);
 }
 
 int cagr_grid_query(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
@@ -520,6 +592,8 @@ int cagr_grid_query(cagr_tree* t, int op, int precision, const void* attrs, int 
     const int64_t total = dims[0] * dims[1] * dims[2];
     if (total == 0) return CAGR_OK;
     hipStream_t st = (hipStream_t)stream;
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
     if (t->n == 0) {
         CAGR_HIP(hipMemsetAsync(out, 0, total * (dtype == CAGR_F32 ? 4 : 8), st));
         return CAGR_OK;
@@ -547,6 +621,8 @@ int cagr_count(cagr_tree* t, int op, const void* attrs, int dtype, const void* w
     CAGR_REQUIRE(attrs, CAGR_ERR_ARG, "null attrs");
     CAGR_REQUIRE(n_far || n_near || n_visit, CAGR_ERR_ARG, "no counter output given");
     hipStream_t st = (hipStream_t)stream;
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
     rc = run_upward_for(t, op, CAGR_PREC_FP64, oi, attrs, dtype, weights, 1.0, st);
     if (rc) return rc;
     Queries q;

@@ -150,6 +150,15 @@ struct cagr_tree {
     int num_cus = 256;               // compute units of the device
     int64_t grid_blocks = 0;         // persistent grid size (0 = from the device)
 
+    // order of the calls on this handle: its workspaces (source and node
+    // records, accumulators, queues, query scratch) serve one call at a time,
+    // so a call enqueued on a stream other than the previous call's first
+    // waits for done_ev, recorded on the calling stream at the end of every
+    // call (not while the stream is being captured into a graph)
+    hipEvent_t done_ev = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool ordered = false;         // done_ev holds the last call's end
+
     // opt-in launch timing (cagr_profile): event pairs per kind, reused
     bool profiling = false;
     std::vector<hipEvent_t> prof_events[2];
@@ -159,6 +168,16 @@ struct cagr_tree {
 namespace cagr {
 // record an event of the current region (begin/end alternate) when profiling
 int prof_mark(cagr_tree* t, int kind, hipStream_t st);
+// call ordering on one handle: call_begin makes `st` wait for the previous
+// call's work when that came on another stream; CallEnd records the end of
+// this call (destructor, so every return path records it)
+int call_begin(cagr_tree* t, hipStream_t st);
+void call_end(cagr_tree* t, hipStream_t st);
+struct CallEnd {
+    cagr_tree* t;
+    hipStream_t st;
+    ~CallEnd() { call_end(t, st); }
+};
 }
 
 namespace cagr {

@@ -365,17 +365,31 @@ CAGR_HD inline bool root_pair(const View& t, const RootInfo& ri, RootInfo& pair)
 #define CAGR_MC_STRIDE 64
 #endif
 constexpr int kStride = CAGR_MC_STRIDE;
+// index check of every workspace access: a no-op in the kernels; the host
+// replay defines it to abort on an index outside the array's length
+#ifndef CAGR_MC_CHECK
+#define CAGR_MC_CHECK(cond) ((void)0)
+#endif
 
-template <class T>
+template <class T, int N>   // N = the array's length in elements
 struct Lanes {
     T* p;
-    CAGR_HD T& operator[](int i) const { return p[(int64_t)i * kStride]; }
+    CAGR_HD T& operator[](int i) const {
+        CAGR_MC_CHECK(i >= 0 && i < N);
+        return p[(int64_t)i * kStride];
+    }
 };
 template <class T, int N>
 struct Vec {   // a small fixed array kept together as one element
     T v[N];
-    CAGR_HD T& operator[](int i) { return v[i]; }
-    CAGR_HD const T& operator[](int i) const { return v[i]; }
+    CAGR_HD T& operator[](int i) {
+        CAGR_MC_CHECK(i >= 0 && i < N);
+        return v[i];
+    }
+    CAGR_HD const T& operator[](int i) const {
+        CAGR_MC_CHECK(i >= 0 && i < N);
+        return v[i];
+    }
 };
 
 constexpr size_t align8(size_t x) { return (x + 7) & ~size_t(7); }
@@ -400,50 +414,50 @@ struct WorkLayout {
     static constexpr size_t bytes = align8(out + sizeof(int32_t) * 3 * kMaxEdges);
 };
 
-template <class T>
-CAGR_HD inline Lanes<T> lanes_at(unsigned char* blk, size_t off, int lane) {
-    return Lanes<T>{reinterpret_cast<T*>(blk + off * kStride) + lane};
+template <class T, int N>
+CAGR_HD inline Lanes<T, N> lanes_at(unsigned char* blk, size_t off, int lane) {
+    return Lanes<T, N>{reinterpret_cast<T*>(blk + off * kStride) + lane};
 }
 
 struct Work {
-    Lanes<int64_t> ea, eb;                     // iso edges (first, second) by root key
+    Lanes<int64_t, kMaxEdges> ea, eb;          // iso edges (first, second) by root key
     int ne = 0;
-    Lanes<int64_t> vkey;                       // vertexCount of the reference
-    Lanes<int> vcount;
-    Lanes<RootInfo> vinfo;
+    Lanes<int64_t, kMaxVerts> vkey;            // vertexCount of the reference
+    Lanes<int, kMaxVerts> vcount;
+    Lanes<RootInfo, kMaxVerts> vinfo;
     int nv = 0;
-    Lanes<int64_t> loop;                       // one loop's vertex keys
-    Lanes<int64_t> back;                       // its back chain while it is built
-    Lanes<int> vid;                            // its vertex numbers
+    Lanes<int64_t, kMaxEdges> loop;            // one loop's vertex keys
+    Lanes<int64_t, kMaxEdges> back;            // its back chain while it is built
+    Lanes<int, kMaxEdges> vid;                 // its vertex numbers
     // triangulation (Geometry.cpp:74-220)
-    Lanes<Vec<float, 3>> tp;
-    Lanes<Vec<int, 2>> te_p, te_t;
-    Lanes<Vec<int, 3>> tt_e;
+    Lanes<Vec<float, 3>, kMaxEdges> tp;
+    Lanes<Vec<int, 2>, 3 * kMaxEdges> te_p, te_t;
+    Lanes<Vec<int, 3>, kMaxEdges> tt_e;
     int nte = 0, ntt = 0;
     bool overflow = false;
     // the leaf's own 12 edge roots, found on first use (0 unknown, 1 found, 2 none)
-    Lanes<int8_t> own_state;
-    Lanes<RootInfo> own;
-    Lanes<int32_t> out;                        // the leaf's triangles, before they are placed
+    Lanes<int8_t, 12> own_state;
+    Lanes<RootInfo, 12> own;
+    Lanes<int32_t, 3 * kMaxEdges> out;         // the leaf's triangles, before they are placed
 
     // lane `lane` of the kStride interleaved workspaces at `blk`
     // (WorkLayout::bytes * kStride bytes, 8-byte aligned)
     CAGR_HD Work(unsigned char* blk, int lane)
-        : ea(lanes_at<int64_t>(blk, WorkLayout::ea, lane)),
-          eb(lanes_at<int64_t>(blk, WorkLayout::eb, lane)),
-          vkey(lanes_at<int64_t>(blk, WorkLayout::vkey, lane)),
-          vcount(lanes_at<int>(blk, WorkLayout::vcount, lane)),
-          vinfo(lanes_at<RootInfo>(blk, WorkLayout::vinfo, lane)),
-          loop(lanes_at<int64_t>(blk, WorkLayout::loop, lane)),
-          back(lanes_at<int64_t>(blk, WorkLayout::back, lane)),
-          vid(lanes_at<int>(blk, WorkLayout::vid, lane)),
-          tp(lanes_at<Vec<float, 3>>(blk, WorkLayout::tp, lane)),
-          te_p(lanes_at<Vec<int, 2>>(blk, WorkLayout::te_p, lane)),
-          te_t(lanes_at<Vec<int, 2>>(blk, WorkLayout::te_t, lane)),
-          tt_e(lanes_at<Vec<int, 3>>(blk, WorkLayout::tt_e, lane)),
-          own_state(lanes_at<int8_t>(blk, WorkLayout::own_state, lane)),
-          own(lanes_at<RootInfo>(blk, WorkLayout::own, lane)),
-          out(lanes_at<int32_t>(blk, WorkLayout::out, lane)) {}
+        : ea(lanes_at<int64_t, kMaxEdges>(blk, WorkLayout::ea, lane)),
+          eb(lanes_at<int64_t, kMaxEdges>(blk, WorkLayout::eb, lane)),
+          vkey(lanes_at<int64_t, kMaxVerts>(blk, WorkLayout::vkey, lane)),
+          vcount(lanes_at<int, kMaxVerts>(blk, WorkLayout::vcount, lane)),
+          vinfo(lanes_at<RootInfo, kMaxVerts>(blk, WorkLayout::vinfo, lane)),
+          loop(lanes_at<int64_t, kMaxEdges>(blk, WorkLayout::loop, lane)),
+          back(lanes_at<int64_t, kMaxEdges>(blk, WorkLayout::back, lane)),
+          vid(lanes_at<int, kMaxEdges>(blk, WorkLayout::vid, lane)),
+          tp(lanes_at<Vec<float, 3>, kMaxEdges>(blk, WorkLayout::tp, lane)),
+          te_p(lanes_at<Vec<int, 2>, 3 * kMaxEdges>(blk, WorkLayout::te_p, lane)),
+          te_t(lanes_at<Vec<int, 2>, 3 * kMaxEdges>(blk, WorkLayout::te_t, lane)),
+          tt_e(lanes_at<Vec<int, 3>, kMaxEdges>(blk, WorkLayout::tt_e, lane)),
+          own_state(lanes_at<int8_t, 12>(blk, WorkLayout::own_state, lane)),
+          own(lanes_at<RootInfo, 12>(blk, WorkLayout::own, lane)),
+          out(lanes_at<int32_t, 3 * kMaxEdges>(blk, WorkLayout::out, lane)) {}
 };
 
 CAGR_HD inline int vfind(const Work& w, int64_t key) {
@@ -586,8 +600,8 @@ CAGR_HD inline int next_loop(Work& w) {
     int64_t front_idx = e0b, back_idx = e0a;
     // front / back chains kept in the loop buffer from both ends
     int nf = 0, nb = 0;
-    const Lanes<int64_t> fr = w.loop;              // front[0..nf) firsts
-    const Lanes<int64_t> back = w.back;
+    const Lanes<int64_t, kMaxEdges> fr = w.loop;              // front[0..nf) firsts
+    const Lanes<int64_t, kMaxEdges> back = w.back;
     for (int j = w.ne - 1; j >= 0; --j) {
         if (w.ea[j] == front_idx || w.eb[j] == front_idx) {
             int64_t first, second;
@@ -736,7 +750,7 @@ CAGR_HD inline int leaf_triangles(const View& t, const Node& leaf, Work& w, VInd
     int count = 0;
     int len;
     while ((len = next_loop(w)) > 0) {
-        const Lanes<int> vid = w.vid;
+        const Lanes<int, kMaxEdges> vid = w.vid;
         int nv = 0;
         for (int j = 0; j < len; ++j) {
             const int v = vindex(w.loop[j]);

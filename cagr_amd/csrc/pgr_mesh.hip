@@ -347,6 +347,14 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     CAGR_HIP(hipMemcpyAsync(&m32, valid.p + n12, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
     const int64_t m = m32;
+    if (g->mesh_v) CAGR_HIP(hipFree(g->mesh_v));
+    if (g->mesh_t) CAGR_HIP(hipFree(g->mesh_t));
+    g->mesh_v = nullptr;
+    g->mesh_t = nullptr;
+    g->n_vertices = g->n_triangles = 0;
+    // no leaf edge crosses the iso-value: the empty mesh (the reference's
+    // GetMCIsoTriangles adds no point and no polygon then)
+    if (m == 0) return CAGR_OK;
     if ((rc = keys.alloc(m)) || (rc = ktmp.alloc(m)) || (rc = occ.alloc(m)) ||
         (rc = otmp.alloc(m)) || (rc = head.alloc(m + 1)) || (rc = hist.alloc(sort_hist_len(m))))
         return rc;
@@ -362,11 +370,6 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     int32_t nv = 0;
     CAGR_HIP(hipMemcpyAsync(&nv, first.p + n12, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
-    if (g->mesh_v) CAGR_HIP(hipFree(g->mesh_v));
-    if (g->mesh_t) CAGR_HIP(hipFree(g->mesh_t));
-    g->mesh_v = nullptr;
-    g->mesh_t = nullptr;
-    g->n_vertices = g->n_triangles = 0;
     CAGR_HIP(hipMalloc(&g->mesh_v, 3 * std::max<int64_t>(nv, 1) * sizeof(float)));
     k_mc_vertices<<<nblk(n12), kT, 0, st>>>(t, g->leaves, first.p, n12, g->mesh_v);
     CAGR_LAUNCHED();
@@ -391,6 +394,11 @@ int pgr_mesh(cagr_pgr* g, const float* values, float iso, float sw, hipStream_t 
     CAGR_HIP(hipMemcpyAsync(&na32, active.p + nl, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
     const int64_t na = na32;
+    if (na == 0) {   // roots without an emitting leaf cannot occur; no triangles then
+        CAGR_HIP(hipStreamSynchronize(st));
+        g->n_vertices = nv;
+        return CAGR_OK;
+    }
     Buf<int32_t> ids;
     if ((rc = ids.alloc(na))) return rc;
     k_compact_ids<<<nblk(nl), kT, 0, st>>>(active.p, nl, ids.p);

@@ -468,6 +468,8 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     t->open_factor = 2.0 / theta;
     t->point_dtype = dtype;
     CAGR_HIP(hipGetDevice(&t->device));
+    // the event that orders later calls on this handle across streams
+    CAGR_HIP(hipEventCreateWithFlags(&t->done_ev, hipEventDisableTiming));
     // clouds: a plain tree is a forest of one
     t->num_clouds = cloud_off ? nclouds : 1;
     t->cloud_pt.assign(t->num_clouds + 1, 0);

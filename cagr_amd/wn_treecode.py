@@ -338,7 +338,8 @@ class WindingNumberTreecode:
         return self.apply("A3", normals, widths, continuous=continuous, precision=precision)
 
     def query(self, queries: torch.Tensor, normals: torch.Tensor, width: float, *, op="A3",
-              continuous=True, precision=None, widths: Optional[torch.Tensor] = None):
+              continuous=True, precision=None, widths: Optional[torch.Tensor] = None,
+              an: float = 1.0):
         """Operator at external query points [Nq, 3] (e.g. GaussRecon's sample pass)."""
         assert normals.shape == (self.points.shape[0], IN_DIM[op])
         queries = queries.contiguous()
@@ -350,12 +351,12 @@ class WindingNumberTreecode:
         with torch.cuda.device(self.device):
             check(self.L.cagr_apply_queries(self._h, OP[op], self._prec(precision), _ptr(normals),
                                             _dt(normals), _ptr(queries), _dt(queries), nq,
-                                            _ptr(widths), float(width), int(continuous), 1.0,
+                                            _ptr(widths), float(width), int(continuous), float(an),
                                             _ptr(out), _stream(self.device)))
         return out
 
     def grid_query(self, normals: torch.Tensor, lo, step, dims, width: float, *, op="A3",
-                   continuous=True, precision=None) -> torch.Tensor:
+                   continuous=True, precision=None, an: float = 1.0) -> torch.Tensor:
         """Operator on the regular grid lo + i*step (i < dims), returned as [d0, d1, d2]."""
         assert normals.shape == self.points.shape
         normals = normals.contiguous()
@@ -367,5 +368,6 @@ class WindingNumberTreecode:
         with torch.cuda.device(self.device):
             check(self.L.cagr_grid_query(self._h, OP[op], self._prec(precision), _ptr(normals),
                                          _dt(normals), lo_a, st_a, dm_a, float(width),
-                                         int(continuous), 1.0, _ptr(out), _stream(self.device)))
+                                         int(continuous), float(an), _ptr(out),
+                                         _stream(self.device)))
         return out

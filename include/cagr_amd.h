@@ -7,6 +7,15 @@
  * thread.  Work is enqueued on the given HIP stream (NULL = legacy default
  * stream) and is asynchronous unless noted.
  *
+ * Streams and threads: calls on one tree handle share its workspaces (per-call
+ * source and node records, accumulators, queues), so they are ordered: a call
+ * enqueued on a stream other than the previous call's makes its stream wait
+ * (hipStreamWaitEvent) for the previous call's work first.  Calls on one
+ * handle therefore never overlap on the device, whatever the streams; calls on
+ * different handles are independent and may run concurrently.  A handle must
+ * not be used from two host threads at once (the ordering state is not
+ * locked).  Calls captured into a HIP graph are ordered by the graph.
+ *
  * Reference interfaces replaced (paths relative to the reference repo root,
  * CUDA-based/ext/wn_treecode/ abbreviated to wnt/):
  *   wnt/wn_treecode_cpu/wn_treecode_cpu_torch_interface.cpp:104-118  build_tree
@@ -37,9 +46,11 @@ extern "C" {
 
 /* precision of the operator arithmetic */
 #define CAGR_PREC_FP64 0     /* plain FP64 throughout (reference formula) */
-#define CAGR_PREC_FP32C 1    /* FP32 kernel sums, two-level compensated (Neumaier)
-                                accumulation; opening tests, cutoff ties and node
-                                moments in FP64 */
+#define CAGR_PREC_FP32C 1    /* FP32 interactions summed in FP32 over short runs (one
+                                leaf's points, one run of far nodes), each run's
+                                partial added into an FP64 running total -- FP64
+                                totals, not Kahan/Neumaier compensation; opening
+                                tests, cutoff ties and node moments in FP64 */
 
 /* operators.  in = attribute dim per point, out = output dim per query.  */
 #define CAGR_OP_A 0          /* in 3, out 1: sum (x-y).mu/r^3 (CUDA sign, cuda_kernels.cu:79-110) */

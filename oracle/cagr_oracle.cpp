@@ -25,6 +25,13 @@
 //     An = 0 reduction (A2k -> A, A2Tk -> AT, A3 -> A), the sphere winding
 //     number closed form, the dense identity A2Tk(An) = (-A2k(-An))^T,
 //     adjoint identities, and the reference-held CAGR output bunny.xyz.
+//   * PARITY UNPINNED: the An-dependent part of the continuous inner branch
+//     of A2k / A3 (inside the width: e^{An w} [d.mu/w^3 + An/2 mu_k/w],
+//     cuda_kernels.cu:140-147, 179-185, 216-222).  Only its An = 0 part
+//     (d.mu/w^3, the CPU core's continuous A negated) is pinned; the factor
+//     e^{An w} and the An/2 mu_k/w term are restated from the source and
+//     checked by nothing reference-derived (the reference holds no output of
+//     its GaussRecon query, and its CUDA kernels cannot be built here).
 //
 // All arithmetic is IEEE double with contraction off (-ffp-contract=off in
 // oracle/Makefile), evaluated in the same operation order as the reference.

@@ -5,6 +5,14 @@
 // result with the reference's own mesh.
 #define CAGR_HD
 #define CAGR_MC_STRIDE 1
+#include <cstdio>
+#include <cstdlib>
+// every workspace index is checked against its array's length here
+#define CAGR_MC_CHECK(cond)                                                          \
+    ((cond) ? (void)0                                                                \
+            : (std::fprintf(stderr, "mc_replay: index out of range: %s (%s:%d)\n",  \
+                            #cond, __FILE__, __LINE__),                              \
+               std::abort()))
 #include <cmath>
 using std::sqrt;
 #include "../../cagr_amd/csrc/mc_logic.h"

@@ -347,3 +347,67 @@ def test_dense_cloud_small_width_matches_oracle(leaf):
         r = wt.forward_A2T_sum(t(s3), t(w), precision=prec).cpu().numpy()[idx]
         assert normwise(a, ref_a) <= tol, (prec, normwise(a, ref_a))
         assert normwise(r, ref_t) <= tol, (prec, normwise(r, ref_t))
+
+
+@pytest.mark.parametrize("an", [0.5, 1.0, 2.0])
+def test_continuous_branch_carries_weight(an):
+    """The continuous inner kernel of A2k / A3 (inside the width:
+    e^{An w} [d.mu/w^3 + An/2 mu_k/w], cuda_kernels.cu:140-147, 179-185,
+    216-222) at several convection strengths, with a width so large that
+    every query has tens of sources inside it: the inner branch then carries
+    a large share of every output (asserted), so a wrong exponent or a dropped
+    An term there cannot hide behind the far field.  Self queries, explicit
+    queries (GaussRecon's sample pass) and a grid, both precisions."""
+    p = synth.normalize(synth.sphere(2500, 21)).astype(np.float32).astype(np.float64)
+    n = p.shape[0]
+    mu = synth.random_normals(n, 22)
+    w = np.full(n, 0.3)
+    wt = WindingNumberTreecode(t(p), leaf_size=8)
+    ot = oracle.build_tree(p, 15, 8)
+    for op in ("A21", "A22", "A23", "A3"):
+        say("continuous branch", an, op)
+        ref = oracle.apply(op, ot, p, mu, w, continuous=True, an=an)
+        outer = oracle.apply(op, ot, p, mu, w, continuous=False, an=an)
+        assert normwise(ref, outer) > 0.2, "the inner branch must carry weight"
+        for prec, tol in (("fp64", TOL64), ("fp32c", TOL32C)):
+            out = wt.apply(op, t(mu), t(w), continuous=True, an=an, precision=prec).cpu().numpy()
+            assert normwise(out, ref) <= tol, (op, prec, normwise(out, ref))
+    fused = oracle.apply("A21", ot, p, mu, w, continuous=True, an=an)
+    out = wt.apply("A2", t(mu), t(w), continuous=True, an=an, precision="fp64").cpu().numpy()
+    assert normwise(out[:, :1], fused) <= TOL64
+    q = np.random.default_rng(23).uniform(-0.6, 0.6, (2000, 3)) + p[:2000]
+    ref = oracle.apply("A3", ot, p, mu, 0.3, queries=q, continuous=True, an=an)
+    for prec, tol in (("fp64", TOL64), ("fp32c", TOL32C)):
+        out = wt.query(t(q), t(mu), 0.3, op="A3", precision=prec, an=an).cpu().numpy()
+        assert normwise(out, ref) <= tol, (prec, normwise(out, ref))
+    dims, lo, step = (9, 11, 10), (-1.0, -1.0, -1.0), (0.25, 0.2, 0.22)
+    ii = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3)
+    gq = np.asarray(lo) + ii * np.asarray(step)
+    ref = oracle.apply("A3", ot, p, mu, 0.3, queries=gq, continuous=True, an=an).reshape(dims)
+    out = wt.grid_query(t(mu), lo, step, dims, 0.3, precision="fp64", an=an).cpu().numpy()
+    assert normwise(out, ref) <= TOL64
+
+
+def test_calls_on_two_streams_are_ordered():
+    """One handle used from two streams with no synchronisation between the
+    calls: every call waits for the previous one's work on the handle's shared
+    records, so the results equal the same calls made on one stream."""
+    p = synth.normalize(synth.ellipsoids(60000, 24)).astype(np.float32)
+    n = p.shape[0]
+    say("two streams", n)
+    wt = WindingNumberTreecode(t(p, torch.float32), leaf_size=32)
+    mus = [t(synth.random_normals(n, 25 + i), torch.float32) for i in range(4)]
+    w = t(np.full(n, 0.002), torch.float32)
+    want = [wt.forward_A2(m, w, precision="fp32c").clone() for m in mus]
+    want_t = [wt.forward_A2T_sum(m, w, precision="fp32c").clone() for m in mus]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    got, got_t = [], []
+    for i, m in enumerate(mus):
+        with torch.cuda.stream(s1 if i % 2 == 0 else s2):
+            got.append(wt.forward_A2(m, w, precision="fp32c"))
+        with torch.cuda.stream(s2 if i % 2 == 0 else s1):
+            got_t.append(wt.forward_A2T_sum(m, w, precision="fp32c"))
+    torch.cuda.synchronize()
+    for a, b in zip(got + got_t, want + want_t):
+        assert torch.equal(a, b)

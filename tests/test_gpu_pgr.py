@@ -141,6 +141,26 @@ def test_pgr_mesh_matches_reference_fixtures(name, tmp_path):
     assert torch.equal(v, v2) and torch.equal(t, t2)
 
 
+def test_pgr_mesh_empty_when_no_corner_crosses_the_iso_value(tmp_path):
+    """An iso-value above every corner value: no edge root, no triangle -- an
+    empty mesh and a valid PLY with zero elements, as the reference's
+    GetMCIsoTriangles + writePolygon2 leave; the octree still meshes after."""
+    from cagr_amd.pcio import save_ply_mesh
+    pts = BUNNY["points"].astype(np.float32)
+    say("pgr mesh empty")
+    g = PGROctree(torch.from_numpy(pts).to(DEV), 8, 1)
+    vals = oracle.sphere_field(g.corners.cpu().numpy())
+    v, t = g.mesh(torch.from_numpy(vals).to(DEV), float(vals.max()) + 1.0, 0.005)
+    assert v.shape == (0, 3) and t.shape == (0, 3)
+    path = str(tmp_path / "empty.ply")
+    save_ply_mesh(path, g.output_vertices(v), t.cpu().numpy())
+    data = open(path, "rb").read()
+    assert b"element vertex 0\n" in data and b"element face 0\n" in data
+    assert data.endswith(b"end_header\n")
+    v2, t2 = g.mesh(torch.from_numpy(vals).to(DEV), 0.0, 0.005)
+    assert v2.shape[0] > 0 and t2.shape[0] > 0
+
+
 def test_gauss_recon_end_to_end(tmp_path):
     """main_GaussRecon_cuda.cu as one call on the bunny's oriented samples: the
     FP64 field differs from the oracle's only in summation order, so the mesh

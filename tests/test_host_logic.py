@@ -1,8 +1,9 @@
 """CPU replay of the HIP tree build (tests/host_logic/build_logic.cpp runs the
 per-element bodies of cagr_amd/csrc/tree_logic.h serially, in the kernels'
 order) against the oracle tree, including degenerate inputs; compiled with
-AddressSanitizer / UBSan on the host so an out-of-bounds index in the shared
-logic fails here rather than on a GPU."""
+UBSan (-fno-sanitize-recover: any report aborts) and libstdc++'s checked
+container indexing (_GLIBCXX_ASSERTIONS), so an out-of-range index in the
+shared logic fails here rather than on a GPU."""
 import ctypes as ct
 import os
 import subprocess
@@ -20,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def hl(tmp_path_factory):
     so = str(tmp_path_factory.mktemp("hl") / "libhl.so")
     subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared", "-fsanitize=undefined",
-                    "-fno-sanitize-recover=all", "-o", so,
+                    "-fno-sanitize-recover=all", "-D_GLIBCXX_ASSERTIONS", "-o", so,
                     os.path.join(HERE, "host_logic", "build_logic.cpp")], check=True)
     L = ct.CDLL(so)
     P, I = ct.c_void_p, ct.c_int64

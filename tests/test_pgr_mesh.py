@@ -32,7 +32,8 @@ def sha(a):
 def replay(tmp_path_factory):
     so = str(tmp_path_factory.mktemp("mc") / "mc_replay.so")
     subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-                    "-fsanitize=undefined", "-o", so,
+                    "-fsanitize=undefined", "-fno-sanitize-recover=all", "-D_GLIBCXX_ASSERTIONS",
+                    "-o", so,
                     os.path.join(HERE, "host_logic", "mc_replay.cpp")], check=True)
     L = ct.CDLL(so)
     L.hl_mesh.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_float, ct.c_float,
@@ -129,3 +130,28 @@ def test_mesh_is_a_closed_surface_for_an_interior_sphere(replay):
     assert len(P) - n_edges + len(T) == 2
     r = np.linalg.norm(P.astype(np.float64) - 0.5, axis=1)
     assert np.abs(r - 0.3).max() < 0.02
+
+
+def test_workspace_index_check_aborts_out_of_range(tmp_path):
+    """The replay's workspace index check is live: one index past an array's
+    length aborts the program (the kernels compile the same accessors with the
+    check off)."""
+    src = tmp_path / "oob.cpp"
+    src.write_text(
+        '#define CAGR_HD\n#define CAGR_MC_STRIDE 1\n#include <cstdio>\n#include <cstdlib>\n'
+        '#define CAGR_MC_CHECK(c) ((c) ? (void)0 : (std::fputs("oob\\n", stderr), std::abort()))\n'
+        '#include <cmath>\nusing std::sqrt;\n'
+        f'#include "{os.path.join(HERE, "..", "cagr_amd", "csrc", "mc_logic.h")}"\n'
+        'using namespace cagr::mc;\n'
+        'alignas(8) static unsigned char blk[WorkLayout::bytes];\n'
+        'int main(int argc, char**) {\n'
+        '    Work w(blk, 0);\n'
+        '    w.ea[kMaxEdges - 1] = 1;\n'
+        '    w.own[11].key = 2;\n'
+        '    if (argc > 1) w.ea[kMaxEdges] = 3;\n'
+        '    return 0;\n}\n')
+    exe = str(tmp_path / "oob")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-o", exe, str(src)], check=True)
+    assert subprocess.run([exe]).returncode == 0
+    r = subprocess.run([exe, "x"], capture_output=True, text=True)
+    assert r.returncode != 0 and "oob" in r.stderr
