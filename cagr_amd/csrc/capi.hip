@@ -4,7 +4,13 @@
 #include "tree_export.h"
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
+
+// scratch of cagr_apply_queries_noalloc (described there)
+constexpr int64_t kNoAllocMaxQueries = int64_t(1) << 21;
+constexpr size_t kNoAllocScratchBytes = size_t(50) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace cagr {
 
@@ -552,6 +558,34 @@ int apply_queries_impl(cagr_tree* t, int op, int precision, const void* attrs, i
     return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
 }
 
+// ---------------------------------------------------------------- no-allocation variant
+// Scratch of cagr_apply_queries_noalloc: the queries' Morton order and its
+// radix-sort buffers, for at most kNoAllocMaxQueries queries (2^21 = a 128^3
+// grid; query_ws_bytes(2^21) = 49.1 MiB).  One copy per GPU, held from the
+// library's load to the end of the process.  In host code `scratch` names a
+// host-side stand-in: its device address comes from hipGetSymbolAddress, per
+// device, and only that address is passed to kernels.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of this device's `scratch` (looked up once per device,
+// with that device current)
+int scratch_address(char** out) {
+    int dev = -1;
+    CAGR_HIP(hipGetDevice(&dev));
+    CAGR_REQUIRE(dev >= 0 && dev < kMaxDevices, CAGR_ERR_STATE, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CAGR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CAGR_REQUIRE(p != nullptr, CAGR_ERR_HIP, "no device address for the query scratch");
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CAGR_OK;
+}
+
 }  // namespace
 }  // namespace cagr
 
@@ -575,6 +609,18 @@ int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, i
     };
     return apply_queries_impl(t, op, precision, attrs, dtype, queries, qdtype, nq, widths, width,
                               continuous, an, out, stream, grow);
+}
+
+int cagr_apply_queries_noalloc(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+                               const void* queries, int qdtype, int64_t nq, const void* widths,
+                               double width, int continuous, double an, void* out, void* stream) {
+    CAGR_REQUIRE(nq <= kNoAllocMaxQueries, CAGR_ERR_ARG,
+                 "cagr_apply_queries_noalloc takes at most 2^21 (2097152) queries");
+    CAGR_REQUIRE(query_ws_bytes(kNoAllocMaxQueries) <= kNoAllocScratchBytes, CAGR_ERR_STATE,
+                 "query scratch smaller than its stated maximum");
+    auto fixed = [](hipStream_t, char** w) -> int { return scratch_address(w); };
+    return apply_queries_impl(t, op, precision, attrs, dtype, queries, qdtype, nq, widths, width,
+                              continuous, an, out, stream, fixed);
 }
 
 int cagr_grid_query(cagr_tree* t, int op, int precision, const void* attrs, int dtype,

@@ -166,6 +166,22 @@ int cagr_apply_queries(cagr_tree* tree, int op, int precision, const void* attrs
                        const void* queries, int qdtype, int64_t nq, const void* widths,
                        double width, int continuous, double an, void* out, void* stream);
 
+/* cagr_apply_queries without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The queries'
+ * Morton order and its sort buffers live in a static per-GPU scratch array
+ * of 50 MiB (held by every GPU the library runs on until the process ends)
+ * instead of the handle's grow-only workspace; nq <= 2^21 (2097152, a 128^3
+ * grid), larger counts are rejected with CAGR_ERR_ARG before any device
+ * work.  Same arguments, kernels and results as cagr_apply_queries.  Every
+ * call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (other handles, other streams, other host threads) race; the scratch keeps
+ * the last call's contents and needs no clearing. */
+int cagr_apply_queries_noalloc(cagr_tree* tree, int op, int precision, const void* attrs,
+                               int dtype, const void* queries, int qdtype, int64_t nq,
+                               const void* widths, double width, int continuous, double an,
+                               void* out, void* stream);
+
 /* Operator on the regular grid lo + i*step, i in [0,dims) per axis (HOST
  * lo/step/dims), generated in-kernel; out [dims0*dims1*dims2] (dtype),
  * row-major with axis 2 fastest.  op in {A, A_CPU, A21..A23, A3}; one scalar
