@@ -31,7 +31,7 @@ EXPORTS = (
     "cagr_last_error", "cagr_version", "cagr_tree_build", "cagr_tree_destroy", "cagr_tree_info",
     "cagr_tree_export", "cagr_tree_permutation", "cagr_upward", "cagr_apply",
     "cagr_apply_weighted", "cagr_apply_adjoint", "cagr_apply_queries",
-    "cagr_apply_queries_noalloc", "cagr_grid_query",
+    "cagr_apply_queries_noalloc", "cagr_apply_moments", "cagr_grid_query",
     "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
     "cagr_order_items", "cagr_profile", "cagr_profile_read", "cagr_adjoint_plan",
     "cagr_apply_adjoint_planned", "cagr_pgr_build", "cagr_pgr_destroy", "cagr_pgr_info",
@@ -53,6 +53,7 @@ _SIGS = {
     "cagr_apply_adjoint": (_C, [_P, _C, _C, _P, _C, _P, _P, _C, _D, _P, _P]),
     "cagr_apply_queries": (_C, [_P, _C, _C, _P, _C, _P, _C, _I, _P, _D, _C, _D, _P, _P]),
     "cagr_apply_queries_noalloc": (_C, [_P, _C, _C, _P, _C, _P, _C, _I, _P, _D, _C, _D, _P, _P]),
+    "cagr_apply_moments": (_C, [_P, _C, _C, _P, _C, _P, _P, _P, _C, _I, _P, _C, _D, _P, _P]),
     "cagr_grid_query": (_C, [_P, _C, _C, _P, _C, _P, _P, _P, _D, _C, _D, _P, _P]),
     "cagr_count": (_C, [_P, _C, _P, _C, _P, _P, _P, _P, _P]),
     "cagr_forest_build": (_C, [_P, _P, _I, _C, _C, _C, _D, _P, ct.POINTER(_P)]),

@@ -514,6 +514,21 @@ __global__ void k_zero_words(uint32_t* p, int64_t n) {
 // query workspace: the Morton order [nq] int32, then the sort's scratch
 size_t query_ws_bytes(int64_t nq) { return nq * sizeof(int32_t) + morton_order_scratch(nq); }
 
+// the handle's grow-only query workspace, at least query_ws_bytes(nq)
+int grow_query_ws(cagr_tree* t, int64_t nq, hipStream_t st, char** w) {
+    const size_t need = query_ws_bytes(nq);
+    if (need > t->ws_queries) {
+        CAGR_HIP(hipStreamSynchronize(st));
+        (void)hipFree(t->qws);
+        t->qws = nullptr;
+        t->ws_queries = 0;
+        CAGR_HIP(hipMalloc(&t->qws, need));
+        t->ws_queries = need;
+    }
+    *w = (char*)t->qws;
+    return CAGR_OK;
+}
+
 // the shared part of cagr_apply_queries and its no-allocation variant; `ws`
 // returns the query workspace (query_ws_bytes(nq) bytes) once the arguments
 // have been checked, or fails
@@ -594,21 +609,61 @@ extern "C" {
 int cagr_apply_queries(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
                        const void* queries, int qdtype, int64_t nq, const void* widths,
                        double width, int continuous, double an, void* out, void* stream) {
-    auto grow = [t, nq](hipStream_t st, char** w) -> int {
-        const size_t need = query_ws_bytes(nq);
-        if (need > t->ws_queries) {   // grow-only query workspace
-            CAGR_HIP(hipStreamSynchronize(st));
-            (void)hipFree(t->qws);
-            t->qws = nullptr;
-            t->ws_queries = 0;
-            CAGR_HIP(hipMalloc(&t->qws, need));
-            t->ws_queries = need;
-        }
-        *w = (char*)t->qws;
-        return CAGR_OK;
-    };
+    auto grow = [t, nq](hipStream_t st, char** w) -> int { return grow_query_ws(t, nq, st, w); };
     return apply_queries_impl(t, op, precision, attrs, dtype, queries, qdtype, nq, widths, width,
                               continuous, an, out, stream, grow);
+}
+
+int cagr_apply_moments(cagr_tree* t, int op, int precision, const void* attrs, int dtype,
+                       const void* node_attrs, const void* node_reppoints, const void* queries,
+                       int qdtype, int64_t nq, const void* widths, int continuous, double an,
+                       void* out, void* stream) {
+    OpInfo oi;
+    int rc = check_common(t, op, precision, dtype, &oi);
+    if (rc) return rc;
+    CAGR_REQUIRE(op != CAGR_OP_A2 && op != CAGR_OP_A2T_SUM, CAGR_ERR_ARG,
+                 "cagr_apply_moments takes the reference's single operators (not A2 / A2T_SUM)");
+    CAGR_REQUIRE(qdtype == CAGR_F32 || qdtype == CAGR_F64, CAGR_ERR_ARG, "bad query dtype");
+    if (!queries) nq = t->n;
+    CAGR_REQUIRE(nq >= 0 && nq < (int64_t(1) << 31) - 1, CAGR_ERR_ARG, "bad query count");
+    if (nq == 0) return CAGR_OK;
+    CAGR_REQUIRE(widths && out, CAGR_ERR_ARG, "null widths / out");
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = call_begin(t, st))) return rc;
+    CallEnd end_{t, st};
+    if (t->n == 0) {
+        const int64_t words = nq * oi.out_dim * (dtype == CAGR_F32 ? 1 : 2);
+        k_zero_words<<<(words + 255) / 256, 256, 0, st>>>((uint32_t*)out, words);
+        CAGR_LAUNCHED();
+        return CAGR_OK;
+    }
+    CAGR_REQUIRE(attrs && node_attrs && node_reppoints, CAGR_ERR_ARG,
+                 "null attrs / node_attrs / node_reppoints");
+    // source records from the point attributes (the upward pass also writes
+    // node records of its own moments; they are replaced by the given ones)
+    rc = run_upward_for(t, op, precision, oi, attrs, dtype, nullptr, an, st);
+    if (rc) return rc;
+    rc = nodes_from_moments(t, oi.family, precision, oi.dir0, node_attrs, node_reppoints, dtype,
+                            oi.convects ? an : 0.0, st);
+    if (rc) return rc;
+    Queries q;
+    q.nq = nq;
+    q.widths = widths;
+    q.wdtype = dtype;
+    if (!queries) {
+        q.mode = 0;
+    } else {
+        char* w = nullptr;
+        if ((rc = grow_query_ws(t, nq, st, &w))) return rc;
+        int32_t* order = (int32_t*)w;
+        rc = morton_order(queries, qdtype, nq, order, w + nq * sizeof(int32_t), st);
+        if (rc) return rc;
+        q.mode = 1;
+        q.q = queries;
+        q.qdtype = qdtype;
+        q.qorder = order;
+    }
+    return traverse(t, op, precision, q, continuous, an, out, dtype, nullptr, nullptr, nullptr, st);
 }
 
 int cagr_apply_queries_noalloc(cagr_tree* t, int op, int precision, const void* attrs, int dtype,

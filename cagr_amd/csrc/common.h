@@ -195,6 +195,8 @@ int scan_exclusive(U* a, int64_t len, U* tmp, hipStream_t st);
 // implemented in upward.hip
 int upward(cagr_tree* t, int family, int precision, int K, int dir0, const void* attrs,
            int dtype, const void* weights, double an, hipStream_t st, int wdtype = -1);
+int nodes_from_moments(cagr_tree* t, int family, int precision, int dir0, const void* node_attrs,
+                       const void* node_reppoints, int dtype, double an, hipStream_t st);
 // implemented in traverse.hip
 struct Queries {
     int mode = 0;                 // 0 self, 1 explicit, 2 grid

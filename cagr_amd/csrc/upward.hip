@@ -401,7 +401,60 @@ int dispatch_upward(cagr_tree* t, int family, int precision, int K, int dir0, co
                : run_upward<TI, 1, 3, CAGR_PREC_FP32C>(t, attrs, weights, wdtype, an, dir0, st);
 }
 
+// node records of one set from caller-given moments (preorder node ids):
+// node_attrs [m, C] and node_reppoints [m, 3] in TI
+template <typename TI, int FAM, int P>
+__global__ __launch_bounds__(kThreads) void k_nodes_from_moments(int64_t m,
+                                                                const TI* __restrict__ attrs,
+                                                                const TI* __restrict__ rep,
+                                                                void* nodes, double an, int dir0) {
+    using R = Rec<FAM, 1, P>;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m) return;
+    double c[3], mm[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) c[d] = (double)rep[n * 3 + d];
+    for (int d = 0; d < R::C; ++d) mm[d] = (double)attrs[n * R::C + d];
+    R::node(nodes, n, 0, c, mm, an, dir0);
+}
+
+template <typename TI>
+int nodes_from_moments_t(cagr_tree* t, int family, int precision, int dir0, const TI* attrs,
+                         const TI* rep, double an, hipStream_t st) {
+    const int64_t m = t->m;
+    if (m == 0) return CAGR_OK;
+    const unsigned nb = (unsigned)blocks_for(m);
+    const bool p64 = precision == CAGR_PREC_FP64;
+    if (family == 0) {
+        if (p64)
+            k_nodes_from_moments<TI, 0, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(m, attrs, rep,
+                                                                               t->nodes, an, dir0);
+        else
+            k_nodes_from_moments<TI, 0, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(m, attrs, rep,
+                                                                                t->nodes, an, dir0);
+    } else {
+        if (p64)
+            k_nodes_from_moments<TI, 1, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(m, attrs, rep,
+                                                                               t->nodes, an, dir0);
+        else
+            k_nodes_from_moments<TI, 1, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(m, attrs, rep,
+                                                                                t->nodes, an, dir0);
+    }
+    CAGR_LAUNCHED();
+    return CAGR_OK;
+}
+
 }  // namespace
+
+// node records of a one-set operator from caller-given moments in `dtype`
+// (replaces the records of the last upward pass; the source records stay)
+int nodes_from_moments(cagr_tree* t, int family, int precision, int dir0, const void* node_attrs,
+                       const void* node_reppoints, int dtype, double an, hipStream_t st) {
+    if (dtype == CAGR_F32)
+        return nodes_from_moments_t<float>(t, family, precision, dir0, (const float*)node_attrs,
+                                           (const float*)node_reppoints, an, st);
+    return nodes_from_moments_t<double>(t, family, precision, dir0, (const double*)node_attrs,
+                                        (const double*)node_reppoints, an, st);
+}
 
 // family 0 = vector (attrs [n,3]), 1 = scalar sets (attrs [n,K]); weights in
 // wdtype (-1: the attributes' dtype)

@@ -158,6 +158,21 @@ int cagr_apply_adjoint_planned(cagr_tree* tree, int op, int precision, const voi
                                const void* widths, int continuous, double an, void* out,
                                void* stream);
 
+/* The reference's multiply_by_X with the node moments it is handed
+ * (wn_treecode_cuda_torch_interface.cu:137-777, wn_treecode_cpu_torch_interface.cpp:
+ * 213-378): the traversal reads node_attrs [M,in(op)] and node_reppoints
+ * [M,3] (`dtype`, preorder node ids -- e.g. the outputs of
+ * scatter_point_attrs_to_nodes) instead of moments of its own upward pass;
+ * attrs [n,in(op)] (original order) feed the leaf sums.  queries [nq,3]
+ * (qdtype) or NULL for the tree's own points (nq is then ignored);
+ * widths [nq] (dtype, per query); out [nq,out(op)] (dtype).  op is one of
+ * the reference's single operators (A, A_CPU, AT, G, A21..A23, A2T1..A2T3,
+ * A3).  The backend of the _cuda / _cpu compatibility modules. */
+int cagr_apply_moments(cagr_tree* tree, int op, int precision, const void* attrs, int dtype,
+                       const void* node_attrs, const void* node_reppoints, const void* queries,
+                       int qdtype, int64_t nq, const void* widths, int continuous, double an,
+                       void* out, void* stream);
+
 /* Operator at external queries [nq,3] (qdtype) -- e.g. the GaussRecon sample
  * isovalue pass (main_GaussRecon_cuda.cu:450-467).  widths [nq] (dtype) or
  * NULL for the scalar `width`.  Queries are Morton-ordered internally for the
