@@ -19,14 +19,17 @@ a bounded sample of queries on the host cores (cpu_baseline).
 
 Solve comparison (solve): CGNR on the normal equations of the linear operator
 A mu = -[A21; A22; A23] mu (unit point weights, the same width), b = 0.5,
-to ||A^T(b - A mu)|| / ||A^T b|| <= --solve-tol, once with the plain FP64
+to ||A^T(b - A mu)|| / ||A^T b|| <= --solve-tol (default 1e-5: the operator's
+condition number, near 1e8, puts 1e-6 some 10^4 iterations away -- minutes
+of FP64 -- so it runs only on request, --solve-tol 1e-6), once with the plain FP64
 operator and exact adjoint, once with FP32c iterations and FP64-verified
 residual replacement (its adjoint through an adjoint plan, whose build is
 timed inside the solve; also reported without the plan); all start from
 mu = 0 on the tree built above.
 
 Seeds (seed_sweep): the same workload on seeds 1..--seeds-1 (new clouds and
-attributes; same setup, steps and solves), with the medians over all seeds.
+attributes; same setup and steps, solves to --sweep-solve-tol), with the
+medians over all seeds.
 ``value`` and the roofline stay those of the seed-0 run (--seed).
 
 Prints one JSON line.
@@ -78,9 +81,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle sample")
-    ap.add_argument("--solve-tol", type=float, default=1e-3,
+    ap.add_argument("--solve-tol", type=float, default=1e-5,
                     help="FP64-verified relative residual of the solve comparison")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--sweep-solve-tol", type=float, default=1e-4,
+                    help="tolerance of the solves of the other seeds (1e-5 takes ~20 s a seed)")
     ap.add_argument("--seeds", type=int, default=5,
                     help="1M config: also time seeds 1..SEEDS-1 (pairs/s, FP64 ratio, solves) "
                          "and report the medians over seeds 0..SEEDS-1")
@@ -116,6 +121,9 @@ def log(*a):
 
 def main():
     args = parse()
+    # the oracle's OpenMP threads stay on their cores (set before any
+    # OpenMP runtime starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
@@ -300,7 +308,8 @@ def main():
     if args.seeds > 1:
         sweep = {"seeds": [args.seed], "fp32c_pairs_per_s": [round(value, 4)],
                  "fp64_pairs_per_s": [round(fp64["value"], 4)] if fp64 else None,
-                 "solve_speedup": [solve["speedup"]] if solve else None}
+                 "solve_speedup": [solve["speedup"]] if solve else None,
+                 "solve_tol": [args.solve_tol, args.sweep_solve_tol] if solve else None}
         del wt
         torch.cuda.empty_cache()
         for sd in range(args.seed + 1, args.seed + args.seeds):
@@ -405,8 +414,8 @@ def seed_run(args, seed, with_fp64, with_solve):
         torch.cuda.synchronize()
         out[prec] = round(steps / (time.perf_counter() - t), 4)
     if with_solve:
-        _, i64 = cgnr_solve(wt, args.width, tol=args.solve_tol)
-        _, i32 = cgnr_solve_mixed(wt, args.width, tol=args.solve_tol)
+        _, i64 = cgnr_solve(wt, args.width, tol=args.sweep_solve_tol)
+        _, i32 = cgnr_solve_mixed(wt, args.width, tol=args.sweep_solve_tol)
         out["solve_speedup"] = round(i64["seconds"] / i32["seconds"], 3)
     log(f"seed {seed}: {out}")
     del wt
@@ -456,6 +465,18 @@ def cpu_baseline(pts, mu_np, s3, args, r32, a32):
             break
         m = int(min(n, max(2 * m, m * args.cpu_seconds / max(dt, 1e-3))))
     per_pair = t_up + dt * n / m
+    # the reference's as-shipped team size (omp_set_num_threads(20),
+    # cpu_kernels.cpp:244) on the same sample, however many cores the host has
+    L.orc_set_num_threads(20)
+    t0 = time.perf_counter()
+    oracle.upward(tree, p64, mu64)
+    for k in range(3):
+        oracle.upward(tree, p64, s3[:, k:k + 1])
+    t_up20 = time.perf_counter() - t0
+    dt20, _, _ = run(q_idx)
+    L.orc_set_num_threads(cores)
+    per_pair20 = t_up20 + dt20 * n / m
+    log(f"cpu: the same sample with 20 threads (the reference's setting): {1 / per_pair20:.5f} pairs/s")
     # accuracy of the GPU FP32c result on the sampled queries against the FP64 oracle
     ga = a32.double().cpu().numpy()[q_idx]
     gr = r32.double().cpu().numpy()[q_idx]
@@ -467,12 +488,36 @@ def cpu_baseline(pts, mu_np, s3, args, r32, a32):
         "value": round(1.0 / per_pair, 6),
         "unit": "matvec pairs/s",
         "cores": cores,
+        "host_cpu": host_cpu(),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "unset"),
+        "as_shipped_20_threads": {"value": round(1.0 / per_pair20, 6),
+                                  "note": f"omp_set_num_threads(20) as cpu_kernels.cpp:244 "
+                                          f"hard-codes it, on {_USABLE_CORES} usable host "
+                                          f"cores"},
         "kind": "port",
         "sample": f"{m} random queries of the same 1M-point tree (all 6 operators), full-size "
                   f"upward passes ({t_up:.2f} s), extrapolated linearly to {n} queries; "
                   f"oracle tree build {t_build:.1f} s not included",
         "gpu_fp32c_err_vs_oracle_on_sample": {"A_mu": ea, "AT_r": er},
     }
+
+
+_USABLE_CORES = len(os.sched_getaffinity(0))   # before any OpenMP team pins this thread
+
+
+def host_cpu() -> str:
+    """The host CPU model (first model name of /proc/cpuinfo) and the cores
+    this process could run on when it started."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}; {_USABLE_CORES} usable cores of {os.cpu_count()}"
 
 
 def _sync_time(fn, reps=1):
@@ -637,11 +682,14 @@ def config_c4(args):
 def config_c5(args):
     """64 clouds (sizes log-uniform in [2e3, 2e5], sphere / torus / ellipsoid,
     sigma 0.001): autotune leaf {16, 32, 64} x theta {0.4, 0.5, 0.6} x waves
-    per block {1, 2, 4, 8}, then the tuned batched persistent form against one
-    single-cloud launch pair per cloud at leaf 32, theta 0.5."""
+    per block {1, 2, 4, 8} under the error budget against direct sums (5e-2,
+    the tuner's default), once for the whole batch and once per size class
+    (< 10k, 10k..60k, >= 60k points); then the tuned batched persistent form,
+    the per-class forests, the batched form at leaf 32 / theta 0.5 and one
+    single-cloud launch pair per cloud at leaf 32 / theta 0.5."""
     import torch
     from cagr_amd import WindingNumberTreecode, synth
-    from cagr_amd.autotune import autotune
+    from cagr_amd.autotune import ClassBatches, autotune, autotune_classes
     from cagr_amd.batched import BatchedTreecode
     dev = torch.device("cuda:0")
     clouds_np = synth.batch_clouds(64, args.seed)
@@ -649,19 +697,27 @@ def config_c5(args):
     mus = [torch.from_numpy(synth.random_normals(c.shape[0], args.seed + i).astype(np.float32)).to(dev)
            for i, c in enumerate(clouds_np)]
     total = int(sum(c.shape[0] for c in clouds_np))
-    rep = autotune(clouds, mus, 0.002, steps=3, log=log)
+    rep = autotune(clouds, mus, 0.002, steps=3, rounds=3, log=log)
     b = rep.best
     log(f"c5 tuned: leaf {b.leaf_size}, theta {b.theta}, waves/block {b.waves_per_block}: "
-        f"{b.ms_per_pair:.3f} ms/pair (error {b.err:.2e}, budget {rep.budget:.2e})")
+        f"{b.ms_per_pair:.3f} ms/pair (error {b.err:.2e}, budget {rep.budget:.2e}, "
+        f"stable {rep.stable}, lead {rep.lead:.1%})")
+    classes = autotune_classes(clouds, mus, 0.002, steps=3, rounds=3, log=log)
     mu = torch.cat(mus).contiguous()
+    w = torch.full((total,), 0.002, dtype=torch.float32, device=dev)
     bt = BatchedTreecode(clouds, leaf_size=b.leaf_size, theta=b.theta, precision="fp32c")
     bt.balance("A2", mu)
     bt.set_launch(b.waves_per_block)
-    w = torch.full((total,), 0.002, dtype=torch.float32, device=dev)
     tuned = _pairs_per_s(bt, mu, w, "fp32c", 10)
+    del bt
+    cb = ClassBatches(clouds, classes)
+    cb.balance(mu)
+    per_class = _pairs_per_s(cb, mu, w, "fp32c", 10)
+    del cb
     bd = BatchedTreecode(clouds, leaf_size=32, theta=0.5, precision="fp32c")
     bd.balance("A2", mu)
     default = _pairs_per_s(bd, mu, w, "fp32c", 10)
+    del bd
     singles = [WindingNumberTreecode(c, leaf_size=32, theta=0.5, precision="fp32c") for c in clouds]
     ws = [torch.full((c.shape[0],), 0.002, dtype=torch.float32, device=dev) for c in clouds]
 
@@ -672,20 +728,29 @@ def config_c5(args):
     all_singles()
     sec, _ = _sync_time(all_singles, 5)
     per_cloud = 1.0 / sec
-    log(f"c5: tuned batched {tuned:.1f} pairs/s, batched leaf 32 {default:.1f}, "
-        f"one launch pair per cloud {per_cloud:.1f}")
+    log(f"c5: tuned batched {tuned:.1f} pairs/s, per-class forests {per_class:.1f}, batched "
+        f"leaf 32 / theta 0.5 {default:.1f}, one launch pair per cloud {per_cloud:.1f}")
     return {"metric": "batched matvec pairs/s (all 64 clouds per pair)",
             "config": {"workload": f"64 clouds, {total} points, w 0.002, A2 + A2T_sum, FP32c",
                        "points": total},
             "data": "synthetic",
             "tuned": {"leaf_size": b.leaf_size, "theta": b.theta,
                       "waves_per_block": b.waves_per_block, "pairs_per_s": round(tuned, 3),
-                      "sampled_err": b.err, "err_budget": rep.budget},
+                      "sampled_err": b.err, "err_budget": rep.budget, "stable": rep.stable,
+                      "lead": round(rep.lead, 4)},
+            "per_class": {"pairs_per_s": round(per_class, 3),
+                          "classes": [{"clouds": int(len(sel)), "leaf_size": r.best.leaf_size,
+                                       "theta": r.best.theta,
+                                       "waves_per_block": r.best.waves_per_block,
+                                       "sampled_err": r.best.err, "stable": r.stable}
+                                      for sel, r in classes]},
             "batched_leaf32_theta05": round(default, 3),
             "per_cloud_launches": round(per_cloud, 3),
+            "speedup_tuned_vs_leaf32_theta05": round(tuned / default, 3),
             "speedup_tuned_vs_per_cloud": round(tuned / per_cloud, 3),
             "table": [[r.leaf_size, r.theta, r.waves_per_block, round(r.ms_per_pair, 4), r.err]
-                      for r in rep.table]}
+                      for r in rep.table],
+            "errors": {f"{k[0]}/{k[1]}": v for k, v in rep.errors.items()}}
 
 
 def config_pgr(args):

@@ -69,6 +69,7 @@ def lib():
         L.orc_adjoint.argtypes = [_I, _I, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P,
                                   _D, _I, _D, _P, _P]
         L.orc_num_threads.restype = _I
+        L.orc_set_num_threads.argtypes = [_I]
         L.pgr_oracle_build.argtypes = [_P, _I, ct.c_int, ct.c_int, _P, _P, _P, _P]
         L.pgr_oracle_fetch.argtypes = [_P, _P]
         _lib = L

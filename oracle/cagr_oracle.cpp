@@ -482,4 +482,8 @@ void orc_adjoint(int64_t op, int64_t nq, const double* q, const double* qw, int6
 
 int64_t orc_num_threads(void) { return omp_get_max_threads(); }
 
+// OpenMP team size of the following parallel loops (the reference's CPU
+// kernels hard-code 20, cpu_kernels.cpp:244; timing runs compare both)
+void orc_set_num_threads(int64_t n) { omp_set_num_threads((int)n); }
+
 }  // extern "C"

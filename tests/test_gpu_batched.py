@@ -135,8 +135,52 @@ def test_autotune_small_batch():
     pts = [torch.from_numpy(c.astype(np.float32)).to(DEV) for c in clouds]
     mus = [torch.from_numpy(synth.random_normals(c.shape[0], 9).astype(np.float32)).to(DEV)
            for c in clouds]
-    rep = autotune(pts, mus, 0.004, waves_per_block=(2, 4), steps=2, samples_per_cloud=32,
-                   log=print)
-    assert rep.best.err <= rep.budget * (1 + 1e-12)
-    assert len(rep.table) == 3 * 3 * 2
-    assert rep.best.ms_per_pair == min(r.ms_per_pair for r in rep.table if r.err <= rep.budget * (1 + 1e-12))
+    rep = autotune(pts, mus, 0.004, waves_per_block=(2, 4), steps=2, rounds=2,
+                   samples_per_cloud=32, log=print)
+    assert len(rep.errors) == 9
+    admissible = {lt for lt, e in rep.errors.items() if e <= rep.budget}
+    assert {(r.leaf_size, r.theta) for r in rep.table} == admissible
+    assert len(rep.table) == 2 * len(admissible)
+    assert rep.best.err <= rep.budget
+    assert rep.best.ms_per_pair == min(r.ms_per_pair for r in rep.table)
+    # a budget nothing meets: the most accurate configuration
+    tight = autotune(pts, mus, 0.004, waves_per_block=(4,), steps=1, rounds=1,
+                     samples_per_cloud=32, budget=1e-12)
+    assert (tight.best.leaf_size, tight.best.theta) == min(tight.errors, key=tight.errors.get)
+
+
+def test_autotune_64_clouds_and_size_classes():
+    """The tuner on the 64-cloud workload: its choice is within the stated
+    budget against direct sums, and the tuned forest -- and the per-size-class
+    forests -- give each cloud what the oracle gives on that cloud's own tree
+    at the tuned leaf size and theta."""
+    from cagr_amd.autotune import ClassBatches, autotune_classes
+    clouds_np = [c.astype(np.float32).astype(np.float64) for c in synth.batch_clouds(64)]
+    pts = [torch.from_numpy(c).to(DEV, torch.float32) for c in clouds_np]
+    mus_np = [synth.random_normals(c.shape[0], 20 + i) for i, c in enumerate(clouds_np)]
+    mus = [torch.from_numpy(m).to(DEV, torch.float32) for m in mus_np]
+    say("autotune 64 clouds")
+    rep = autotune(pts, mus, 0.002, waves_per_block=(2, 8), steps=2, rounds=2,
+                   samples_per_cloud=16, log=print)
+    b = rep.best
+    print(f"[tuned] leaf {b.leaf_size} theta {b.theta} wpb {b.waves_per_block} "
+          f"{b.ms_per_pair:.3f} ms err {b.err:.2e} stable {rep.stable}", flush=True)
+    assert b.err <= rep.budget
+    classes = autotune_classes(pts, mus, 0.002, waves_per_block=(4,), steps=1, rounds=1,
+                               samples_per_cloud=16)
+    assert sorted(np.concatenate([sel for sel, _ in classes]).tolist()) == list(range(64))
+    cb = ClassBatches(pts, classes)
+    mu = torch.cat(mus)
+    w = torch.full((mu.shape[0],), 0.002, device=DEV)
+    out = cb.forward_A2(mu, w, precision="fp64").double().cpu().numpy()
+    offs = np.concatenate([[0], np.cumsum([c.shape[0] for c in clouds_np])])
+    for sel, r in classes:
+        for c in sel[:: max(1, len(sel) // 4)]:       # a few clouds of every class
+            p = clouds_np[c]
+            ot = oracle.build_tree(p, 15, r.best.leaf_size)
+            idx = np.sort(np.random.default_rng(c).choice(p.shape[0], 32, replace=False))
+            mu32 = mus_np[c].astype(np.float32).astype(np.float64)
+            ref = np.concatenate([oracle.apply(f"A2{k}", ot, p, mu32, 0.002, queries=p[idx],
+                                               theta=r.best.theta) for k in (1, 2, 3)], 1)
+            got = out[offs[c]:offs[c + 1]][idx]
+            assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max(), c
