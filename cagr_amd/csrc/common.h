@@ -54,8 +54,12 @@ template <int K> struct alignas(16) SrcS64 { double y[3]; double s[K]; };
 template <int K> struct alignas(16) SrcS32 { float y[3]; float a[K]; };
 //   a[k] = s_k * exp(-An*y_{dir_k}/2)   (= s_k for AT)
 struct alignas(16) NodeS64 { double c[3]; double s; };
-struct alignas(16) NodeS32 { double c[3]; float se; float pad; };
-//   se = s * exp(-An*c_{dir}/2) at this set's centre (the product in FP64)
+struct alignas(16) NodeS32 { float cf[3]; float se; float hi2, lo2, ce; float pad; };
+//   se = s * exp(-An*c_{dir}/2) at this set's centre (the product in FP64);
+//   The FP32 walk decides far/near on cf = (float)c: an FP32 distance^2
+//   above hi2 is far and below lo2 near for the FP64 test too; between them
+//   (or for the cutoff, within ce of the width) the lane retests in FP64 with
+//   the upward pass's r64 entry (open_band)
 
 // persistent traversal queues: one per XCD, counters a cache line apart
 constexpr int kQueues = 8;

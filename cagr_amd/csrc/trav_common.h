@@ -42,6 +42,22 @@ __device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, d
     return in;
 }
 
+// r >= w for a far node decided as the FP64 reference would, from the FP32
+// distance^2 to the node's float centre: r2 carries the centre's rounding
+// (ce per component, NodeV32) besides its own, so the band where FP32 cannot
+// call it is widened by 4 w ce + 4 ce^2; lanes in the band retest with the
+// FP64 centre c (read only then)
+__device__ inline bool inside_centre(float r2, float w2, float wf, float ce, const double* x,
+                                     const double* c, double w) {
+    const float band = fmaf(4.f * wf, ce, fmaf(4.f * ce, ce, 1e-5f * w2));
+    const bool amb = fabsf(r2 - w2) <= band;
+    bool in = r2 < w2;
+    if (__builtin_expect(__ballot(amb) != 0, 0)) {
+        if (amb) in = !(sqrt(dist2_rn(x[0] - c[0], x[1] - c[1], x[2] - c[2])) >= w);
+    }
+    return in;
+}
+
 // sum of v over the 64 lanes of the wave (every lane gets the total); all
 // lanes must be active
 __device__ inline float wave_sum(float v) {
@@ -96,6 +112,12 @@ __device__ inline bool leaf_surely_outside(const float* xf, float4 lb, float wf)
     const float lim = lb.w + wf * 1.0001f + 1e-6f;
     return d2 * (1.f - 1e-6f) > lim * lim;
 }
+
+// Make the listed wave-uniform values available at this point: the scalar
+// loads of a node step's records are issued together and waited for once,
+// instead of each where a branch first uses it (one scalar-memory round trip
+// per step instead of several)
+#define CAGR_LOADS_HERE(...) asm volatile("" ::__VA_ARGS__)
 
 // Element i of a record array addressed by a 32-bit byte offset: a scalar
 // load then takes base + SGPR offset (one multiply) instead of 64-bit address

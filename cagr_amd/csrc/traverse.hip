@@ -70,6 +70,8 @@ struct TP {
     double open_factor;       // 2 / theta
     const float4* lbound;     // leaf bounding spheres
     const int32_t* tile_order;// regular grid: dispatch slot -> wave (null: identity)
+    const double* r64;        // FP64 centres of the upward pass, [set][node][3] (retests)
+    int64_t nm;               // nodes (set stride of r64)
 };
 
 using namespace dev;
@@ -528,39 +530,82 @@ __device__ __forceinline__ void trav_s(
             NodeS32 nrs[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) nrs[k] = rec32(nodes, n * K + k);
+            CAGR_LOADS_HERE("s"(t.escape), "s"(t.info));
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                CAGR_LOADS_HERE("s"(nrs[k].cf[0]), "s"(nrs[k].cf[1]), "s"(nrs[k].cf[2]),
+                                "s"(nrs[k].se), "s"(nrs[k].hi2), "s"(nrs[k].lo2));
             const bool safe = topo_depth(t) <= dsafe;
+            // far interaction of set k (FP32 d and r2 to its float centre)
+            auto far_set = [&](int k, const NodeS32& nr, const float* d, float r2) {
+                bool in = false;   // far at depth <= dsafe proves r > w
+                if (!safe)
+                    in = inside_centre(r2, w2, wf, nr.ce, x, p.r64 + 3 * ((int64_t)k * p.nm + n), w);
+                const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+                const float gi = eq[k] * nr.se * ir;   // g / r
+                const float hg = hanf * gi;
+                const float gi3 = gi * (ir * ir);      // g / r^3
+                for (int cc = 0; cc < 3; ++cc) fp[cc] = fmaf(gi3, d[cc], fp[cc]);
+                if constexpr (K == 3) {
+                    fp[k] += hg;
+                } else {
+                    for (int cc = 0; cc < 3; ++cc) fp[cc] += (cc == p.kdir) ? hg : 0.f;
+                }
+            };
+            auto dist_set = [&](const NodeS32& nr, float* d) {
+                d[0] = xf[0] - nr.cf[0];
+                d[1] = xf[1] - nr.cf[1];
+                d[2] = xf[2] - nr.cf[2];
+                if constexpr (QM != QM_SELF)
+                    for (int cc = 0; cc < 3; ++cc) d[cc] += xl[cc];
+                return fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+            };
             any_near = false;
+            bool amb[K];
+            bool any_amb = false;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
+                // FP32 distance to the float centre: far above the record's
+                // hi2, near below lo2; lanes between them wait for the FP64
+                // test after the loop (open_band)
                 const NodeS32& nr = nrs[k];
                 const bool act = resume[k] <= n;
-                const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
-                const double d2 = dist2_rn(e0, e1, e2);
-                const bool far = d2 > t.thr2;
+                float d[3];
+                const float r2 = dist_set(nr, d);
+                const bool far = r2 > nr.hi2;
+                amb[k] = act & !far & !(r2 < nr.lo2);
+                any_amb |= amb[k];
                 if constexpr (CNT) nvisit += act ? 1 : 0;
-                if (act && far) {
+                if (act & far) {
                     resume[k] = t.escape;
-                    if constexpr (CNT) {
-                        ++nfar;
-                    } else {
-                        const float d[3] = {(float)e0, (float)e1, (float)e2};
-                        const float r2 = (float)d2;
-                        bool in = false;   // far at depth <= dsafe proves r > w
-                        if (!safe) in = inside_fp32(r2, w2, e0, e1, e2, w);
-                        const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
-                        const float gi = eq[k] * nr.se * ir;   // g / r
-                        const float hg = hanf * gi;
-                        const float gi3 = gi * (ir * ir);      // g / r^3
-                        for (int cc = 0; cc < 3; ++cc) fp[cc] = fmaf(gi3, d[cc], fp[cc]);
-                        if constexpr (K == 3) {
-                            fp[k] += hg;
+                    if constexpr (CNT) ++nfar;
+                    else far_set(k, nr, d, r2);
+                }
+                nearl[k] = act & !far & !amb[k];
+                any_near |= nearl[k];
+            }
+            if (__builtin_expect(__ballot(any_amb) != 0, 0)) {
+                // the FP64 test with the upward pass's FP64 centre, then the
+                // same far / near work as above
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (amb[k]) {
+                        const double* c64 = p.r64 + 3 * ((int64_t)k * p.nm + n);
+                        if (dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > t.thr2) {
+                            resume[k] = t.escape;
+                            if constexpr (CNT) {
+                                ++nfar;
+                            } else {
+                                float d[3];
+                                const float r2 = dist_set(nrs[k], d);
+                                far_set(k, nrs[k], d, r2);
+                            }
                         } else {
-                            for (int cc = 0; cc < 3; ++cc) fp[cc] += (cc == p.kdir) ? hg : 0.f;
+                            nearl[k] = true;
+                            any_near = true;
                         }
                     }
                 }
-                nearl[k] = act && !far;
-                any_near |= nearl[k];
             }
             if (topo_leaf(t)) break;
             n = __ballot(any_near) == 0 ? t.escape : n + 1;
@@ -1042,6 +1087,8 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     tp.cnt_visit = cnt_visit;
     tp.open_factor = t->open_factor;
     tp.lbound = t->lbound;
+    tp.r64 = t->r64;
+    tp.nm = t->m;
     int64_t lanes;
     if (q.mode == QM_GRID) {
         for (int d = 0; d < 3; ++d) {

@@ -42,6 +42,45 @@ __device__ inline double pick3(const double* y, int i) {
     return i == 0 ? y[0] : (i == 1 ? y[1] : y[2]);
 }
 
+// float bounds of a double, rounded outwards
+__device__ inline float up_float(double x) {
+    const float f = (float)x;
+    return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+__device__ inline float down_float(double x) {
+    const float f = (float)x;
+    return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
+
+// The FP32 scalar-family walk decides far/near on the float centre cf =
+// (float)c of each set (NodeS32): it forms
+// d = (x_f - cf) (+ x_lo, the low part of an FP64 query split into floats)
+// and r2 = fma(d0, d0, fma(d1, d1, d2 * d2)).  Per component d is then off by
+// at most E = |c - cf| + 2^-23 (|d| + |c - cf|) + 2^-46 (|c| + 2 |d|) (two
+// FP32 roundings and the split of the query), so at |d| = thr
+// |r2 - dist^2| <= B = 2 sqrt(3) thr E + 3 E^2 + 2^-22 thr^2 (r2's own
+// roundings) + 2^-50 thr^2 (the FP64 reference's rounding of its dist^2).
+// The error grows more slowly than |dist^2 - thr^2| away from the threshold,
+// so r2 > hi2 = thr^2 + 2B proves the FP64 test far and r2 < lo2 = thr^2 - 2B
+// proves it near; lanes between them retest in FP64.  ce = |c - cf| +
+// 2^-46 (|c| + 2) is the centre's part of E, which the walks' cutoff test
+// (r >= w) widens its own band by.
+__device__ inline void open_band(const double* c, double thr2, float* cf, float& hi2,
+                                 float& lo2, float& ce) {
+    double delta = 0.0, cmax = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        cf[d] = (float)c[d];
+        delta = fmax(delta, fabs(c[d] - (double)cf[d]));
+        cmax = fmax(cmax, fabs(c[d]));
+    }
+    const double thr = sqrt(thr2);
+    const double E = delta + 0x1p-23 * (thr + delta) + 0x1p-46 * (cmax + 2.0 * thr);
+    const double B = 3.4641016151377546 * thr * E + 3.0 * E * E + (0x1p-22 + 0x1p-50) * thr2;
+    hi2 = up_float(thr2 + 2.0 * B);
+    lo2 = down_float(thr2 - 2.0 * B);
+    ce = up_float(delta + 0x1p-46 * (cmax + 2.0));
+}
+
 // record writers --------------------------------------------------------------
 template <int FAM, int K, int P>
 struct Rec;
@@ -51,7 +90,7 @@ template <int P>
 struct Rec<0, 1, P> {
     static constexpr int C = 3;
     __device__ static void node(void* nodes, int64_t n, int k, const double* c, const double* mm,
-                                double an, int /*dir0*/) {
+                                double an, int /*dir0*/, double /*thr2*/) {
         if constexpr (P == CAGR_PREC_FP64) {
             NodeV64 r;
             for (int d = 0; d < 3; ++d) { r.c[d] = c[d]; r.m[d] = mm[d]; }
@@ -97,7 +136,7 @@ template <int K, int P>
 struct Rec<1, K, P> {
     static constexpr int C = 1;
     __device__ static void node(void* nodes, int64_t n, int k, const double* c, const double* mm,
-                                double an, int dir0) {
+                                double an, int dir0, double thr2) {
         if constexpr (P == CAGR_PREC_FP64) {
             NodeS64 r;
             for (int d = 0; d < 3; ++d) r.c[d] = c[d];
@@ -105,7 +144,7 @@ struct Rec<1, K, P> {
             ((NodeS64*)nodes)[n * K + k] = r;
         } else {
             NodeS32 r;
-            for (int d = 0; d < 3; ++d) r.c[d] = c[d];
+            open_band(c, thr2, r.cf, r.hi2, r.lo2, r.ce);
             r.se = (float)(mm[0] * exp(-0.5 * an * pick3(c, dir0 + k)));
             r.pad = 0.f;
             ((NodeS32*)nodes)[n * K + k] = r;
@@ -171,7 +210,7 @@ __device__ inline void leaf_finish(int32_t nd, const TopoRec& t, const double (&
         w64[o] = tw[k];
         for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
         for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[k][cc];
-        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0);
+        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2);
     }
 }
 
@@ -230,7 +269,7 @@ __device__ inline void up_internal_node(int32_t nd, const TopoRec& t, const int3
         w64[o] = tw[k];
         for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
         for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[k][cc];
-        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0);
+        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2);
     }
 }
 
@@ -403,18 +442,24 @@ int dispatch_upward(cagr_tree* t, int family, int precision, int K, int dir0, co
 
 // node records of one set from caller-given moments (preorder node ids):
 // node_attrs [m, C] and node_reppoints [m, 3] in TI
+// (r64, set 0, gets the same centres: the FP32 walks' FP64 retests read it)
 template <typename TI, int FAM, int P>
 __global__ __launch_bounds__(kThreads) void k_nodes_from_moments(int64_t m,
                                                                 const TI* __restrict__ attrs,
                                                                 const TI* __restrict__ rep,
-                                                                void* nodes, double an, int dir0) {
+                                                                const TopoRec* __restrict__ topo,
+                                                                void* nodes, double* r64,
+                                                                double an, int dir0) {
     using R = Rec<FAM, 1, P>;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m) return;
     double c[3], mm[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < 3; ++d) c[d] = (double)rep[n * 3 + d];
+    for (int d = 0; d < 3; ++d) {
+        c[d] = (double)rep[n * 3 + d];
+        r64[3 * n + d] = c[d];
+    }
     for (int d = 0; d < R::C; ++d) mm[d] = (double)attrs[n * R::C + d];
-    R::node(nodes, n, 0, c, mm, an, dir0);
+    R::node(nodes, n, 0, c, mm, an, dir0, topo[n].thr2);
 }
 
 template <typename TI>
@@ -426,18 +471,18 @@ int nodes_from_moments_t(cagr_tree* t, int family, int precision, int dir0, cons
     const bool p64 = precision == CAGR_PREC_FP64;
     if (family == 0) {
         if (p64)
-            k_nodes_from_moments<TI, 0, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(m, attrs, rep,
-                                                                               t->nodes, an, dir0);
+            k_nodes_from_moments<TI, 0, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
         else
-            k_nodes_from_moments<TI, 0, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(m, attrs, rep,
-                                                                                t->nodes, an, dir0);
+            k_nodes_from_moments<TI, 0, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
     } else {
         if (p64)
-            k_nodes_from_moments<TI, 1, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(m, attrs, rep,
-                                                                               t->nodes, an, dir0);
+            k_nodes_from_moments<TI, 1, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
         else
-            k_nodes_from_moments<TI, 1, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(m, attrs, rep,
-                                                                                t->nodes, an, dir0);
+            k_nodes_from_moments<TI, 1, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
     }
     CAGR_LAUNCHED();
     return CAGR_OK;
