@@ -19,9 +19,10 @@ a bounded sample of queries on the host cores (cpu_baseline).
 
 Solve comparison (solve): CGNR on the normal equations of the linear operator
 A mu = -[A21; A22; A23] mu (unit point weights, the same width), b = 0.5,
-to ||A^T(b - A mu)|| / ||A^T b|| <= --solve-tol (default 1e-5: the operator's
-condition number, near 1e8, puts 1e-6 some 10^4 iterations away -- minutes
-of FP64 -- so it runs only on request, --solve-tol 1e-6), once with the plain FP64
+to ||A^T(b - A mu)|| / ||A^T b|| <= --solve-tol (default 1e-6, the metric's
+tolerance: the operator's condition number, near 1e8, puts it about 10^4
+CG iterations away, some 2.5 minutes of FP64; both solves restart CG from
+the FP64 residual when it stalls), once with the plain FP64
 operator and exact adjoint, once with FP32c iterations and FP64-verified
 residual replacement (its adjoint through an adjoint plan, whose build is
 timed inside the solve; also reported without the plan); all start from
@@ -81,9 +82,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU time of the oracle sample")
-    ap.add_argument("--solve-tol", type=float, default=1e-5,
+    ap.add_argument("--solve-tol", type=float, default=1e-6,
                     help="FP64-verified relative residual of the solve comparison")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solve-fp64-plain", action="store_true",
+                    help="also time the FP64 solve as one CG run without restarts")
     ap.add_argument("--solve-max-iter", type=int, default=40000,
                     help="iteration cap of the solve comparison (both solves)")
     ap.add_argument("--sweep-solve-tol", type=float, default=1e-4,
@@ -280,10 +283,19 @@ def main():
         log(f"solve: fp64 CGNR to {args.solve_tol:g}")
         mu64, i64 = cgnr_solve(wt, args.width, tol=args.solve_tol, max_iter=args.solve_max_iter,
                                log=log)
-        log(f"solve: fp64 {i64['iters']} iterations, {i64['seconds']:.3f} s, "
+        log(f"solve: fp64 {i64['iters']} iterations ({i64['restarts']} restarts), {i64['seconds']:.3f} s, "
             f"residual {i64['rel_residual']:.3e}")
         mu32, i32 = cgnr_solve_mixed(wt, args.width, tol=args.solve_tol,
                                      max_iter=args.solve_max_iter, log=log)
+        plain = None
+        if args.solve_fp64_plain:
+            log(f"solve: fp64 CGNR to {args.solve_tol:g}, one CG run (no restarts)")
+            _, ipl = cgnr_solve(wt, args.width, tol=args.solve_tol, max_iter=args.solve_max_iter,
+                                log=log, stall=0)
+            plain = {"iters": ipl["iters"], "seconds": round(ipl["seconds"], 4),
+                     "rel_residual": ipl["rel_residual"],
+                     "speedup_of_mixed": round(ipl["seconds"] / i32["seconds"], 3)}
+            log(f"solve: fp64 without restarts {ipl['iters']} iterations, {ipl['seconds']:.3f} s")
         log(f"solve: mixed {i32['iters']} iterations ({i32['restarts']} restarts), "
             f"{i32['seconds']:.3f} s (adjoint plan built inside), residual {i32['rel_residual']:.3e}")
         # the adjoint plan's worth, at the sweep tolerance (a shorter solve)
@@ -294,8 +306,9 @@ def main():
         dmu = float(torch.linalg.norm(mu32 - mu64) / torch.linalg.norm(mu64))
         solve = {
             "tol": args.solve_tol,
-            "fp64": {"iters": i64["iters"], "seconds": round(i64["seconds"], 4),
-                     "rel_residual": i64["rel_residual"]},
+            "fp64": {"iters": i64["iters"], "restarts": i64["restarts"],
+                     "seconds": round(i64["seconds"], 4), "rel_residual": i64["rel_residual"]},
+            "fp64_one_cg_run": plain,
             "mixed": {"iters": i32["iters"], "restarts": i32["restarts"],
                       "seconds": round(i32["seconds"], 4), "rel_residual": i32["rel_residual"]},
             "speedup": round(i64["seconds"] / i32["seconds"], 3),

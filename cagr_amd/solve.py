@@ -11,11 +11,11 @@
   equations A^T A mu = A^T b of the linear operator A mu = -[A21 mu; A22 mu;
   A23 mu] (node centres fixed, unit point weights, one width), with the exact
   adjoint kernel for A^T, FP64 vectors and scalars, stopping when
-  ||A^T(b - A mu)|| / ||A^T b|| <= tol in FP64.
+  ||A^T(b - A mu)|| / ||A^T b|| <= tol in FP64; CG restarts from the true
+  residual when it stalls.
 * ``cgnr_solve_mixed`` -- the same solve with the FP32-compensated operator
-  doing the iterations (all CG vectors, dot products and updates FP64) and the
-  stopping test redone with the FP64 operator, restarting from the true
-  residual if the FP64 test fails.
+  doing the iterations (all CG vectors, dot products and updates FP64), the
+  stopping test and every restart computed with the FP64 operator.
 """
 from __future__ import annotations
 
@@ -155,35 +155,12 @@ def _cg(op: NormalOperator, rhs, x0, tol_abs, max_iter, precision, log=None, log
     return x, it
 
 
-def cgnr_solve(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
-               max_iter: int = 5000, an: float = 1.0, log=None):
-    """Plain FP64 CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol."""
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    op = NormalOperator(wt, width, an)
-    n = wt.points.shape[0]
-    bt = torch.full((n, 3), b, dtype=torch.float64, device=wt.device)
-    rhs = op.AT(bt, "fp64")
-    nrhs = float(torch.linalg.norm(rhs))
-    mu, iters = _cg(op, rhs, torch.zeros_like(rhs), tol * nrhs, max_iter, "fp64", log)
-    true_res = float(torch.linalg.norm(op.AT(bt - op.A(mu, "fp64"), "fp64"))) / nrhs
-    torch.cuda.synchronize()
-    return mu, {"iters": iters, "rel_residual": true_res, "seconds": time.perf_counter() - t0,
-                "calls": dict(op.calls)}
-
-
-def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
-                     max_iter: int = 5000, max_restarts: int = 10, an: float = 1.0,
-                     plan: bool = True, log=None, stall: int = 1500):
-    """CGNR whose operator applications run in FP32c while every vector, dot
-    product and update stays FP64; when the recurrence converges -- or
-    stagnates (``stall`` iterations without a 10 % gain: the FP32c forward and
-    adjoint are transposes of each other only to FP32 rounding, which puts a
-    floor near 2e-6 under CG at 1M points) -- the residual is recomputed with
-    the FP64 operator and exact adjoint and CG restarts on the correction from
-    that true residual (iterative refinement; the correction is small, so its
-    floor is lower) until the FP64 test passes.  ``plan``: the FP32c adjoint
-    runs through an adjoint plan (its build is inside the timed region)."""
+def _cgnr(wt: WindingNumberTreecode, width: float, tol: float, b: float, max_iter: int,
+          max_restarts: int, an: float, inner: str, plan: bool, log, stall: int):
+    """CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol, verified with the FP64
+    operator: CG on the correction equation with the ``inner`` operator until
+    it converges or stalls (``stall`` iterations without a 10 % gain; 0 =
+    never), then the true residual in FP64 and a restart from it."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     op = NormalOperator(wt, width, an, plan=plan)
@@ -199,7 +176,7 @@ def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6,
         history.append(res)
         if res <= tol or restarts > max_restarts or iters >= max_iter:
             break
-        d, it = _cg(op, r, torch.zeros_like(r), tol * nrhs, max_iter - iters, "fp32c", log,
+        d, it = _cg(op, r, torch.zeros_like(r), tol * nrhs, max_iter - iters, inner, log,
                     stall=stall)
         iters += it
         mu += d
@@ -208,3 +185,26 @@ def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6,
     torch.cuda.synchronize()
     return mu, {"iters": iters, "restarts": restarts - 1, "rel_residual": res, "history": history,
                 "seconds": time.perf_counter() - t0, "calls": dict(op.calls)}
+
+
+def cgnr_solve(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
+               max_iter: int = 5000, an: float = 1.0, log=None, stall: int = 1500,
+               max_restarts: int = 10):
+    """Plain FP64 CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol, with the same
+    restart rule as the mixed solve (``stall`` = 0: one CG run)."""
+    return _cgnr(wt, width, tol, b, max_iter, max_restarts, an, "fp64", False, log, stall)
+
+
+def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
+                     max_iter: int = 5000, max_restarts: int = 10, an: float = 1.0,
+                     plan: bool = True, log=None, stall: int = 1500):
+    """CGNR whose operator applications run in FP32c while every vector, dot
+    product and update stays FP64; when the recurrence converges -- or
+    stagnates (``stall`` iterations without a 10 % gain: the FP32c forward and
+    adjoint are transposes of each other only to FP32 rounding, which puts a
+    floor near 2e-6 under CG at 1M points) -- the residual is recomputed with
+    the FP64 operator and exact adjoint and CG restarts on the correction from
+    that true residual (iterative refinement; the correction is small, so its
+    floor is lower) until the FP64 test passes.  ``plan``: the FP32c adjoint
+    runs through an adjoint plan (its build is inside the timed region)."""
+    return _cgnr(wt, width, tol, b, max_iter, max_restarts, an, "fp32c", plan, log, stall)
