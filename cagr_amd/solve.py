@@ -118,40 +118,60 @@ def _dot(a, b):
     return float((a * b).sum())
 
 
-def _cg(op: NormalOperator, rhs, x0, tol_abs, max_iter, precision, log=None, log_every=500,
-        stall: int = 0):
+def _cg(op, rhs, x0, tol_abs, max_iter, precision, log=None, log_every=500, stall: int = 0,
+        check_every: int = 10):
     """CG on A^T A x = rhs (FP64 vectors); stops on the recurrence residual.
-    ``stall`` > 0: also stops once the residual has not fallen below 0.9 of
-    its best value for that many iterations (an operator whose forward and
-    adjoint are not exact transposes of each other -- the FP32c pair -- makes
-    CG stagnate near its precision floor; the caller restarts from the true
-    residual).  ``log(msg)`` (optional) gets the recurrence residual every
-    ``log_every`` iterations."""
+    The scalars alpha and beta stay on the device and the host reads the
+    residual history only every ``check_every`` iterations, so the host never
+    waits for the GPU inside an iteration (a run may take up to
+    ``check_every`` - 1 iterations more than it needs; they only lower the
+    residual).  ``stall`` > 0: also stops once the residual has not fallen
+    below 0.9 of its best value for that many iterations (an operator whose
+    forward and adjoint are not exact transposes of each other -- the FP32c
+    pair -- makes CG stagnate near its precision floor; the caller restarts
+    from the true residual).  ``log(msg)`` (optional) gets the recurrence
+    residual every ``log_every`` iterations."""
     x = x0.clone()
     r = rhs - op.normal(x, precision) if x0.abs().max() > 0 else rhs.clone()
     p = r.clone()
-    rr = _dot(r, r)
+    rv, pv = r.view(-1), p.view(-1)
+    rr = torch.dot(rv, rv)
+    hist = torch.empty(check_every, dtype=rr.dtype, device=rr.device)
     it = 0
-    best, since = rr, 0
-    while it < max_iter and rr ** 0.5 > tol_abs:
-        q = op.normal(p, precision)
-        alpha = rr / _dot(p, q)
-        x += alpha * p
-        r -= alpha * q
-        rr_new = _dot(r, r)
-        p = r + (rr_new / rr) * p
-        rr = rr_new
-        it += 1
-        if log is not None and it % log_every == 0:
-            log(f"cg ({precision}) iteration {it}: residual {rr ** 0.5:.3e} (target {tol_abs:.3e})")
-        if rr < 0.81 * best:
-            best, since = rr, 0
-        else:
-            since += 1
-            if stall and since >= stall:
-                if log is not None:
-                    log(f"cg ({precision}) stalled at iteration {it}: residual {rr ** 0.5:.3e}")
-                break
+    best, since = float(rr), 0
+    if best ** 0.5 <= tol_abs:
+        return x, 0
+    while it < max_iter:
+        n = min(check_every, max_iter - it)
+        for j in range(n):
+            q = op.normal(p, precision)
+            pq = torch.dot(pv, q.view(-1))
+            alpha = torch.where(pq != 0, rr / pq, torch.zeros_like(pq))
+            x.addcmul_(p, alpha)
+            r.addcmul_(q, alpha, value=-1.0)
+            rr_new = torch.dot(rv, rv)
+            p.mul_(rr_new / rr).add_(r)
+            rr = rr_new
+            hist[j] = rr
+        rrs = hist[:n].tolist()     # the one host read of the block
+        done = False
+        for j, v in enumerate(rrs):
+            k = it + j + 1
+            if log is not None and k % log_every == 0:
+                log(f"cg ({precision}) iteration {k}: residual {v ** 0.5:.3e} (target {tol_abs:.3e})")
+            if v ** 0.5 <= tol_abs:
+                done = True
+            elif v < 0.81 * best:
+                best, since = v, 0
+            else:
+                since += 1
+                if stall and since >= stall and not done:
+                    if log is not None:
+                        log(f"cg ({precision}) stalled at iteration {k}: residual {v ** 0.5:.3e}")
+                    done = True
+        it += n
+        if done:
+            break
     return x, it
 
 
