@@ -272,6 +272,10 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         n = __builtin_amdgcn_readfirstlane(n);
         const TopoRec t = rec32(topo, n);
         const NodeV32 nr = rec32(nodes, n);
+        // the step's records loaded together and waited for once
+        CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
+                        "s"(nr.c[2]), "s"(nr.e[0]), "s"(nr.e[1]), "s"(nr.e[2]));
+        __builtin_amdgcn_sched_barrier(0);
         const bool act = resume <= n;
         const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
         const double d2 = dist2_rn(e0, e1, e2);
@@ -395,6 +399,9 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
         const bool take = (m >> lane) & 1;
         const TopoRec t = rec32(topo, n);
         const NodeV32 nr = rec32(nodes, n);
+        CAGR_LOADS_HERE("s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]), "s"(nr.c[2]), "s"(nr.e[0]),
+                        "s"(nr.e[1]), "s"(nr.e[2]));
+        __builtin_amdgcn_sched_barrier(0);
         const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
         const double d2 = dist2_rn(e0, e1, e2);
         const float d[3] = {(float)e0, (float)e1, (float)e2};
@@ -526,6 +533,9 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
         n = __builtin_amdgcn_readfirstlane(n);
         const TopoRec t = rec32(topo, n);
         const NodeV64 nr = rec32(nodes, n);
+        CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
+                        "s"(nr.c[2]));
+        __builtin_amdgcn_sched_barrier(0);
         const bool act = resume <= n;
         const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
         const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;

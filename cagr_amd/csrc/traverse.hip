@@ -133,6 +133,18 @@ struct VAcc32 {
     __device__ void add(float* dst, const float* d, float r2, bool in, const float* mu,
                         const float* e, const float* q) {
         const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+        const float dmu = add_ir(dst, d, ir, mu, e, q);
+        if constexpr (CONT) {
+            if (in) {
+                P += dmu;
+                for (int k = 0; k < 3; ++k) Q[k] += mu[k];
+            }
+        }
+    }
+    // the same with 1/r given (0 inside the width) and no continuous-kernel
+    // sums; returns d.mu
+    __device__ float add_ir(float* dst, const float* d, float ir, const float* mu, const float* e,
+                            const float* q) {
         const float ir2 = ir * ir;
         const float ir3 = ir2 * ir;
         const float dmu = fmaf(d[0], mu[0], fmaf(d[1], mu[1], d[2] * mu[2]));
@@ -145,12 +157,7 @@ struct VAcc32 {
             const float t = dmu * ir3;
             for (int k = 0; k < 3; ++k) dst[k] = fmaf(e[k], t, fmaf(q[k], ir, dst[k]));
         }
-        if constexpr (CONT) {
-            if (in) {
-                P += dmu;
-                for (int k = 0; k < 3; ++k) Q[k] += mu[k];
-            }
-        }
+        return dmu;
     }
     // the leaf partial and the far partial since the last leaf join the FP64
     // total (the pair sum is exact in FP64, one rounding per fold)
@@ -267,6 +274,12 @@ __device__ __forceinline__ void trav_v(
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = rec32(topo, n);
             const NodeV32 nr = rec32(nodes, n);
+            // the step's records loaded together and waited for once (the far
+            // interaction's fields would otherwise be a second round trip)
+            CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
+                            "s"(nr.c[2]), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]), "s"(nr.e[0]),
+                            "s"(nr.e[1]), "s"(nr.e[2]), "s"(nr.q[0]), "s"(nr.q[1]), "s"(nr.q[2]));
+            __builtin_amdgcn_sched_barrier(0);
             const bool act = resume <= n;
             const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
             const double d2 = dist2_rn(e0, e1, e2);
@@ -279,9 +292,19 @@ __device__ __forceinline__ void trav_v(
                 } else {
                     const float d[3] = {(float)e0, (float)e1, (float)e2};
                     const float r2 = (float)d2;
-                    bool in = false;   // far at depth <= dsafe proves r > w
-                    if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w);
-                    acc.add(acc.fp, d, r2, in, nr.m, nr.e, nr.q);
+                    // far at depth <= dsafe proves r > w: the cutoff test (and
+                    // its select) only below that depth
+                    if constexpr (CONT) {
+                        bool in = false;
+                        if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w);
+                        acc.add(acc.fp, d, r2, in, nr.m, nr.e, nr.q);
+                    } else {
+                        float ir = __builtin_amdgcn_rsqf(r2);
+                        if (topo_depth(t) > dsafe) {
+                            if (inside_fp32(r2, w2, e0, e1, e2, w)) ir = 0.f;
+                        }
+                        acc.add_ir(acc.fp, d, ir, nr.m, nr.e, nr.q);
+                    }
                 }
             }
             const bool nearl = act && !far;
@@ -360,6 +383,10 @@ __device__ __forceinline__ void trav_v(
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = rec32(topo, n);
             const NodeV64 nr = rec32(nodes, n);
+            // the step's records loaded together and waited for once
+            CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
+                            "s"(nr.c[2]), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]));
+            __builtin_amdgcn_sched_barrier(0);
             const bool act = resume <= n;
             const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
             const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
@@ -530,18 +557,31 @@ __device__ __forceinline__ void trav_s(
             NodeS32 nrs[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) nrs[k] = rec32(nodes, n * K + k);
-            CAGR_LOADS_HERE("s"(t.escape), "s"(t.info));
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                CAGR_LOADS_HERE("s"(nrs[k].cf[0]), "s"(nrs[k].cf[1]), "s"(nrs[k].cf[2]),
-                                "s"(nrs[k].se), "s"(nrs[k].hi2), "s"(nrs[k].lo2));
+            // one wait for the whole step's records
+            if constexpr (K == 3) {
+                CAGR_LOADS_HERE("s"(t.escape), "s"(t.info),
+                                "s"(nrs[0].cf[0]), "s"(nrs[0].cf[1]), "s"(nrs[0].cf[2]),
+                                "s"(nrs[0].se), "s"(nrs[0].hi2), "s"(nrs[0].lo2),
+                                "s"(nrs[1].cf[0]), "s"(nrs[1].cf[1]), "s"(nrs[1].cf[2]),
+                                "s"(nrs[1].se), "s"(nrs[1].hi2), "s"(nrs[1].lo2),
+                                "s"(nrs[2].cf[0]), "s"(nrs[2].cf[1]), "s"(nrs[2].cf[2]),
+                                "s"(nrs[2].se), "s"(nrs[2].hi2), "s"(nrs[2].lo2));
+            } else {
+                CAGR_LOADS_HERE("s"(t.escape), "s"(t.info),
+                                "s"(nrs[0].cf[0]), "s"(nrs[0].cf[1]), "s"(nrs[0].cf[2]),
+                                "s"(nrs[0].se), "s"(nrs[0].hi2), "s"(nrs[0].lo2));
+            }
+            __builtin_amdgcn_sched_barrier(0);
             const bool safe = topo_depth(t) <= dsafe;
             // far interaction of set k (FP32 d and r2 to its float centre)
             auto far_set = [&](int k, const NodeS32& nr, const float* d, float r2) {
-                bool in = false;   // far at depth <= dsafe proves r > w
-                if (!safe)
-                    in = inside_centre(r2, w2, wf, nr.ce, x, p.r64 + 3 * ((int64_t)k * p.nm + n), w);
-                const float ir = in ? 0.f : __builtin_amdgcn_rsqf(r2);
+                // far at depth <= dsafe proves r > w: the cutoff test (and its
+                // select) only below that depth
+                float ir = __builtin_amdgcn_rsqf(r2);
+                if (!safe) {
+                    if (inside_centre(r2, w2, wf, nr.ce, x, p.r64 + 3 * ((int64_t)k * p.nm + n), w))
+                        ir = 0.f;
+                }
                 const float gi = eq[k] * nr.se * ir;   // g / r
                 const float hg = hanf * gi;
                 const float gi3 = gi * (ir * ir);      // g / r^3
@@ -721,9 +761,24 @@ __device__ __forceinline__ void trav_s(
             for (;;) {
             n = __builtin_amdgcn_readfirstlane(n);
             t = rec32(topo, n);
+            NodeS64 nrs[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) nrs[k] = rec32(nodes, n * K + k);
+            // the step's records loaded together and waited for once
+            if constexpr (K == 3) {
+                CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info),
+                                "s"(nrs[0].c[0]), "s"(nrs[0].c[1]), "s"(nrs[0].c[2]), "s"(nrs[0].s),
+                                "s"(nrs[1].c[0]), "s"(nrs[1].c[1]), "s"(nrs[1].c[2]), "s"(nrs[1].s),
+                                "s"(nrs[2].c[0]), "s"(nrs[2].c[1]), "s"(nrs[2].c[2]), "s"(nrs[2].s));
+            } else {
+                CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info),
+                                "s"(nrs[0].c[0]), "s"(nrs[0].c[1]), "s"(nrs[0].c[2]), "s"(nrs[0].s));
+            }
+            __builtin_amdgcn_sched_barrier(0);
             any_near = false;
+#pragma unroll
             for (int k = 0; k < K; ++k) {
-                const NodeS64 nr = rec32(nodes, n * K + k);
+                const NodeS64& nr = nrs[k];
                 const bool act = resume[k] <= n;
                 const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
                 const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
