@@ -119,6 +119,65 @@ __device__ inline bool leaf_surely_outside(const float* xf, float4 lb, float wf)
 // per step instead of several)
 #define CAGR_LOADS_HERE(...) asm volatile("" ::__VA_ARGS__)
 
+// Two source records' fields available here: their scalar loads are issued
+// together and waited for once (a scalar load is waited for with
+// lgkmcnt(0), so the compiler would otherwise wait for the first record
+// before issuing the second's loads)
+__device__ __forceinline__ void loads_here(const SrcV32& a, const SrcV32& b) {
+    CAGR_LOADS_HERE("s"(a.y[0]), "s"(a.y[1]), "s"(a.y[2]), "s"(a.mu[0]), "s"(a.mu[1]),
+                    "s"(a.mu[2]), "s"(a.e[0]), "s"(a.e[1]), "s"(a.e[2]), "s"(a.q[0]), "s"(a.q[1]),
+                    "s"(a.q[2]), "s"(b.y[0]), "s"(b.y[1]), "s"(b.y[2]), "s"(b.mu[0]), "s"(b.mu[1]),
+                    "s"(b.mu[2]), "s"(b.e[0]), "s"(b.e[1]), "s"(b.e[2]), "s"(b.q[0]), "s"(b.q[1]),
+                    "s"(b.q[2]));
+}
+template <int K>
+__device__ __forceinline__ void loads_here(const SrcS32<K>& a, const SrcS32<K>& b) {
+    if constexpr (K == 3) {
+        CAGR_LOADS_HERE("s"(a.y[0]), "s"(a.y[1]), "s"(a.y[2]), "s"(a.a[0]), "s"(a.a[1]),
+                        "s"(a.a[2]), "s"(b.y[0]), "s"(b.y[1]), "s"(b.y[2]), "s"(b.a[0]),
+                        "s"(b.a[1]), "s"(b.a[2]));
+    } else {
+        static_assert(K == 1, "one or three sets");
+        CAGR_LOADS_HERE("s"(a.y[0]), "s"(a.y[1]), "s"(a.y[2]), "s"(a.a[0]), "s"(b.y[0]),
+                        "s"(b.y[1]), "s"(b.y[2]), "s"(b.a[0]));
+    }
+}
+
+__device__ __forceinline__ void loads_here(const SrcV64& a, const SrcV64& b) {
+    CAGR_LOADS_HERE("s"(a.y[0]), "s"(a.y[1]), "s"(a.y[2]), "s"(a.mu[0]), "s"(a.mu[1]),
+                    "s"(a.mu[2]), "s"(b.y[0]), "s"(b.y[1]), "s"(b.y[2]), "s"(b.mu[0]),
+                    "s"(b.mu[1]), "s"(b.mu[2]));
+}
+template <int K>
+__device__ __forceinline__ void loads_here(const SrcS64<K>& a, const SrcS64<K>& b) {
+    if constexpr (K == 3) {
+        CAGR_LOADS_HERE("s"(a.y[0]), "s"(a.y[1]), "s"(a.y[2]), "s"(a.s[0]), "s"(a.s[1]),
+                        "s"(a.s[2]), "s"(b.y[0]), "s"(b.y[1]), "s"(b.y[2]), "s"(b.s[0]),
+                        "s"(b.s[1]), "s"(b.s[2]));
+    } else {
+        static_assert(K == 1, "one or three sets");
+        CAGR_LOADS_HERE("s"(a.y[0]), "s"(a.y[1]), "s"(a.y[2]), "s"(a.s[0]), "s"(b.y[0]),
+                        "s"(b.y[1]), "s"(b.y[2]), "s"(b.s[0]));
+    }
+}
+
+// f(rec) for the wave-uniform records arr[b..e) in order, two at a time, the
+// two records' loads waited for once
+template <typename R, typename F>
+__device__ __forceinline__ void for_each_rec(const R* __restrict__ arr, int32_t b, int32_t e,
+                                             F&& f) {
+    int32_t j = b;
+    for (; j + 1 < e; j += 2) {
+        const R r0 = arr[j];
+        const R r1 = arr[j + 1];
+        loads_here(r0, r1);
+        __builtin_amdgcn_sched_barrier(0);
+        f(r0);
+        f(r1);
+    }
+    if (j < e) f(arr[j]);
+}
+
 // Element i of a record array addressed by a 32-bit byte offset: a scalar
 // load then takes base + SGPR offset (one multiply) instead of 64-bit address
 // arithmetic on every node step.  The host guarantees count * sizeof(T) < 2^32.

@@ -341,17 +341,15 @@ __device__ __forceinline__ void trav_v(
                         if (nearl) acc.flush_leaf();
                     } else if (nearl) {
                         if (fast) {   // no pair of this leaf is within any lane's width
-                            for (int32_t j = t.begin; j < t.end; ++j) {
-                                const SrcV32 sr = src[j];
+                            for_each_rec(src, t.begin, t.end, [&](const SrcV32& sr) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
                                 const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
                                 acc.add(acc.lp, d, r2, false, sr.mu, sr.e, sr.q);
-                            }
+                            });
                         } else {
-                            for (int32_t j = t.begin; j < t.end; ++j) {
-                                const SrcV32 sr = src[j];
+                            for_each_rec(src, t.begin, t.end, [&](const SrcV32& sr) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
@@ -360,7 +358,7 @@ __device__ __forceinline__ void trav_v(
                                                             x[1] - (double)sr.y[1],
                                                             x[2] - (double)sr.y[2], w);
                                 acc.add(acc.lp, d, r2, in, sr.mu, sr.e, sr.q);
-                            }
+                            });
                         }
                         acc.flush_leaf();
                     }
@@ -433,11 +431,10 @@ __device__ __forceinline__ void trav_v(
                             }
                         }
                     } else if (nearl) {
-                        for (int32_t j = t.begin; j < t.end; ++j) {
-                            const SrcV64 sr = src[j];
+                        for_each_rec(src, t.begin, t.end, [&](const SrcV64& sr) {
                             const double dd[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
                             acc.add(dd, w, sr.mu, p.an);
-                        }
+                        });
                     }
                 }
             }
@@ -697,17 +694,15 @@ __device__ __forceinline__ void trav_s(
 #pragma unroll
                         for (int k = 0; k < K; ++k) mk[k] = nearl[k] ? eq[k] : 0.f;
                         if (fast) {   // no pair of this leaf is within any lane's width
-                            for (int32_t j = t.begin; j < t.end; ++j) {
-                                const SrcS32<K> sr = src[j];
+                            for_each_rec(src, t.begin, t.end, [&](const SrcS32<K>& sr) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
                                 const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
                                 near_pair(lp, sr, d, r2, false, mk);
-                            }
+                            });
                         } else {
-                            for (int32_t j = t.begin; j < t.end; ++j) {
-                                const SrcS32<K> sr = src[j];
+                            for_each_rec(src, t.begin, t.end, [&](const SrcS32<K>& sr) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
@@ -716,7 +711,7 @@ __device__ __forceinline__ void trav_s(
                                                             x[1] - (double)sr.y[1],
                                                             x[2] - (double)sr.y[2], w);
                                 near_pair(lp, sr, d, r2, in, mk);
-                            }
+                            });
                         }
                         for (int cc = 0; cc < 3; ++cc) {
                             s[cc] += (double)lp[cc] + (double)fp[cc];
@@ -825,12 +820,11 @@ __device__ __forceinline__ void trav_s(
                             if (any_near) acc[cc] += v;
                         }
                     } else if (any_near) {
-                        for (int32_t j = t.begin; j < t.end; ++j) {
-                            const SrcS64<K> sr = src[j];
+                        for_each_rec(src, t.begin, t.end, [&](const SrcS64<K>& sr) {
                             const double d[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
                             for (int k = 0; k < K; ++k)
                                 if (nearl[k]) add(d, sr.s[k], p.kdir + k);
-                        }
+                        });
                     }
                 }
                 n = t.escape;
