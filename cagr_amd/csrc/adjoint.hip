@@ -300,8 +300,8 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                 if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, takefar);
                 float z[3] = {0.f, 0.f, 0.f};
                 adj_term32<GA, CONT>(z, d, r2, takefar, in, h, nr.e, hanf, gin, winv, ew);
-                const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]),
-                            s2 = wave_sum_dpp(z[2]);
+                float s0, s1, s2;
+                wave_sum3(z[0], z[1], z[2], s0, s1, s2);
                 if (lane == 0) {
                     atomicAdd(p.adj_node + 3 * (int64_t)n, (double)s0);
                     atomicAdd(p.adj_node + 3 * (int64_t)n + 1, (double)s1);
@@ -410,7 +410,8 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
         if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, take);
         float z[3] = {0.f, 0.f, 0.f};
         adj_term32<GA, CONT>(z, d, r2, take, in, h, nr.e, hanf, gin, winv, ew);
-        const float s0 = wave_sum_dpp(z[0]), s1 = wave_sum_dpp(z[1]), s2 = wave_sum_dpp(z[2]);
+        float s0, s1, s2;
+        wave_sum3(z[0], z[1], z[2], s0, s1, s2);
         if (lane == 0) p.far_val[pos] = make_float4(s0, s1, s2, 0.f);
     }
     const int32_t near0 = __builtin_amdgcn_readfirstlane(p.near_off[wave]);
@@ -447,9 +448,7 @@ __global__ __launch_bounds__(kThreads) void k_far_gather(int64_t m, const int32_
         z1 += (double)v.y;
         z2 += (double)v.z;
     }
-    z0 = wave_sum(z0);
-    z1 = wave_sum(z1);
-    z2 = wave_sum(z2);
+    wave_sum3(z0, z1, z2, z0, z1, z2);
     if (lane == 0) {
         adj_node[3 * nd] = z0;
         adj_node[3 * nd + 1] = z1;
@@ -546,7 +545,8 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
             adj_term64<GA, CONT>(z, d, w, gk, p.an, p.sign);
         }
         if (__ballot(takefar) != 0) {
-            const double s0 = wave_sum(z[0]), s1 = wave_sum(z[1]), s2 = wave_sum(z[2]);
+            double s0, s1, s2;
+            wave_sum3(z[0], z[1], z[2], s0, s1, s2);
             if (lane == 0) {
                 atomicAdd(p.adj_node + 3 * (int64_t)n, s0);
                 atomicAdd(p.adj_node + 3 * (int64_t)n + 1, s1);

@@ -2,6 +2,8 @@
 #pragma once
 #include "common.h"
 
+#include <type_traits>
+
 namespace cagr {
 namespace dev {
 
@@ -58,33 +60,72 @@ __device__ inline bool inside_centre(float r2, float w2, float wf, float ce, con
     return in;
 }
 
-// sum of v over the 64 lanes of the wave (every lane gets the total); all
-// lanes must be active
-__device__ inline float wave_sum(float v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-__device__ inline double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
-// wave-wide sum through DPP lane moves (no LDS traffic): quad swaps, row
-// mirrors, then the two row broadcasts; the total ends in lane 63 and is
-// returned to every lane.  All 64 lanes must be active.
+// DPP lane move of a float within rows of 16 lanes (no LDS traffic)
 template <int CTRL, int ROWMASK>
 __device__ inline float dpp_move(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
                                                                  ROWMASK, 0xf, false));
 }
-__device__ inline float wave_sum_dpp(float v) {
-    v += dpp_move<0xb1, 0xf>(v);    // quad_perm [1,0,3,2]
-    v += dpp_move<0x4e, 0xf>(v);    // quad_perm [2,3,0,1]
-    v += dpp_move<0x141, 0xf>(v);   // row_half_mirror
-    v += dpp_move<0x140, 0xf>(v);   // row_mirror: every lane holds its row's sum
-    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
-    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+
+// the sums over the wave's 64 lanes of three values, as scalars: the lane
+// halves are exchanged with v_permlane32_swap (z0 with z1, z2 with itself)
+// and added, the rows of 16 lanes with v_permlane16_swap, then each row is
+// reduced by DPP moves -- row 0 then holds z0's total, row 1 z2's, row 2
+// z1's.  13 instructions against 21 for three separate DPP reductions.  All
+// 64 lanes must be active.
+__device__ inline void wave_sum3(float z0, float z1, float z2, float& s0, float& s1, float& s2) {
+    auto bits = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto flt = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto a = __builtin_amdgcn_permlane32_swap(bits(z0), bits(z1), false, false);
+    const float u = flt(a[0]) + flt(a[1]);   // lanes 0-31: z0[l] + z0[l+32]; 32-63: z1's
+    const auto b = __builtin_amdgcn_permlane32_swap(bits(z2), bits(z2), false, false);
+    const float v = flt(b[0]) + flt(b[1]);   // every lane l: z2[l % 32] + z2[l % 32 + 32]
+    const auto c = __builtin_amdgcn_permlane16_swap(bits(u), bits(v), false, false);
+    float w = flt(c[0]) + flt(c[1]);         // rows: u 0+1, v 0+1, u 2+3, v 2+3
+    w += dpp_move<0xb1, 0xf>(w);    // quad_perm [1,0,3,2]
+    w += dpp_move<0x4e, 0xf>(w);    // quad_perm [2,3,0,1]
+    w += dpp_move<0x141, 0xf>(w);   // row_half_mirror
+    w += dpp_move<0x140, 0xf>(w);   // row_mirror: every lane holds its row's sum
+    s0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, w), 0));
+    s2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, w), 16));
+    s1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, w), 32));
+}
+
+// the same for doubles: each exchange moves the two 32-bit halves, each DPP
+// step moves both halves and adds in FP64
+__device__ inline void wave_sum3(double z0, double z1, double z2, double& s0, double& s1,
+                                 double& s2) {
+    auto lo = [](double v) { return (unsigned)(__builtin_bit_cast(uint64_t, v) & 0xffffffffu); };
+    auto hi = [](double v) { return (unsigned)(__builtin_bit_cast(uint64_t, v) >> 32); };
+    auto dbl = [](unsigned l, unsigned h) {
+        return __builtin_bit_cast(double, ((uint64_t)h << 32) | (uint64_t)l);
+    };
+    const auto al = __builtin_amdgcn_permlane32_swap(lo(z0), lo(z1), false, false);
+    const auto ah = __builtin_amdgcn_permlane32_swap(hi(z0), hi(z1), false, false);
+    const double u = dbl(al[0], ah[0]) + dbl(al[1], ah[1]);
+    const auto bl = __builtin_amdgcn_permlane32_swap(lo(z2), lo(z2), false, false);
+    const auto bh = __builtin_amdgcn_permlane32_swap(hi(z2), hi(z2), false, false);
+    const double v = dbl(bl[0], bh[0]) + dbl(bl[1], bh[1]);
+    const auto cl = __builtin_amdgcn_permlane16_swap(lo(u), lo(v), false, false);
+    const auto ch = __builtin_amdgcn_permlane16_swap(hi(u), hi(v), false, false);
+    double w = dbl(cl[0], ch[0]) + dbl(cl[1], ch[1]);
+    auto move = [&](auto ctrl) {
+        constexpr int C = decltype(ctrl)::value;
+        const unsigned l = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo(w), C, 0xf, 0xf, false);
+        const unsigned h = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi(w), C, 0xf, 0xf, false);
+        return dbl(l, h);
+    };
+    w += move(std::integral_constant<int, 0xb1>{});
+    w += move(std::integral_constant<int, 0x4e>{});
+    w += move(std::integral_constant<int, 0x141>{});
+    w += move(std::integral_constant<int, 0x140>{});
+    s0 = dbl((unsigned)__builtin_amdgcn_readlane((int)lo(w), 0),
+             (unsigned)__builtin_amdgcn_readlane((int)hi(w), 0));
+    s2 = dbl((unsigned)__builtin_amdgcn_readlane((int)lo(w), 16),
+             (unsigned)__builtin_amdgcn_readlane((int)hi(w), 16));
+    s1 = dbl((unsigned)__builtin_amdgcn_readlane((int)lo(w), 32),
+             (unsigned)__builtin_amdgcn_readlane((int)hi(w), 32));
 }
 
 __device__ inline double wave_max(double v) {
