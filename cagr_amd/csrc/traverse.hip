@@ -280,12 +280,16 @@ __device__ __forceinline__ void trav_v(
                             "s"(nr.c[2]), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]), "s"(nr.e[0]),
                             "s"(nr.e[1]), "s"(nr.e[2]), "s"(nr.q[0]), "s"(nr.q[1]), "s"(nr.q[2]));
             __builtin_amdgcn_sched_barrier(0);
-            const bool act = resume <= n;
             const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
             const double d2 = dist2_rn(e0, e1, e2);
-            const bool far = d2 > t.thr2;
-            if constexpr (CNT) nvisit += act ? 1 : 0;
-            if (act && far) {
+            // each comparison as its own wave mask, combined on the scalar
+            // unit (a ballot of a combined lane predicate, or a predicate
+            // carried past the far branch, would go through a vector register)
+            const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= n);
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 > t.thr2);
+            const uint64_t near_m = am & ~fm;
+            if constexpr (CNT) nvisit += (resume <= n) ? 1 : 0;
+            if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
                 resume = t.escape;
                 if constexpr (CNT) {
                     ++nfar;
@@ -307,16 +311,16 @@ __device__ __forceinline__ void trav_v(
                     }
                 }
             }
-            const bool nearl = act && !far;
             const bool leaf = topo_leaf(t);
             if (leaf) {
+                const bool nearl = __builtin_amdgcn_inverse_ballot_w64(near_m);
                 if (nearl) resume = t.escape;
                 if constexpr (CNT) {
                     if (nearl) nnear += t.end - t.begin;
-                } else if (__ballot(nearl) != 0) {
+                } else if (near_m != 0) {
                     const bool sure = leaf_surely_outside(xf, p.lbound[n], wf);
                     const bool fast = __ballot(nearl && !sure) == 0;
-                    const uint64_t nm = __ballot(nearl);
+                    const uint64_t nm = near_m;
                     const int lr = fast ? spread_log2(__popcll(nm), t.end - t.begin) : 0;
                     if (lr > 0) {   // few near lanes: the idle lanes share the points
                         const Spread sp = spread_setup(nm, nearl, lr);
@@ -366,7 +370,7 @@ __device__ __forceinline__ void trav_v(
             }
             // one latch: past a leaf or a subtree no lane enters, the escape;
             // else the first child
-            n = (leaf || __ballot(nearl) == 0) ? t.escape : n + 1;
+            n = (leaf || near_m == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             acc.flush_leaf();
@@ -544,10 +548,12 @@ __device__ __forceinline__ void trav_s(
         for (int32_t n = root; n < stop;) {
             // internal nodes until a leaf or the end of the tree (this inner
             // loop leaves the leaf-only state alone, so the register allocator
-            // keeps its loop-carried values in place), then the leaf
+            // keeps its loop-carried values in place), then the leaf.  The
+            // near lanes of each set travel as wave masks (scalar registers):
+            // per-lane booleans carried out of the loop would be kept as
+            // 0/1 values in vector registers and converted back on every step
             TopoRec t;
-            bool nearl[K];
-            bool any_near;
+            uint64_t nm[K];
             for (;;) {
             n = __builtin_amdgcn_readfirstlane(n);
             t = rec32(topo, n);
@@ -597,57 +603,70 @@ __device__ __forceinline__ void trav_s(
                     for (int cc = 0; cc < 3; ++cc) d[cc] += xl[cc];
                 return fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
             };
-            any_near = false;
-            bool amb[K];
-            bool any_amb = false;
+            uint64_t amb[K];
+            uint64_t any_amb = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 // FP32 distance to the float centre: far above the record's
-                // hi2, near below lo2; lanes between them wait for the FP64
-                // test after the loop (open_band)
+                // hi2, near below lo2 (lo2 < hi2); lanes between them wait for
+                // the FP64 test after the loop (open_band)
                 const NodeS32& nr = nrs[k];
-                const bool act = resume[k] <= n;
                 float d[3];
                 const float r2 = dist_set(nr, d);
-                const bool far = r2 > nr.hi2;
-                amb[k] = act & !far & !(r2 < nr.lo2);
+                // each comparison as its own wave mask, combined on the
+                // scalar unit (a ballot of a combined lane predicate would be
+                // taken through a vector register)
+                const uint64_t am = __builtin_amdgcn_ballot_w64(resume[k] <= n);
+                const uint64_t fm = __builtin_amdgcn_ballot_w64(r2 > nr.hi2);
+                const uint64_t lm = __builtin_amdgcn_ballot_w64(r2 < nr.lo2);
+                nm[k] = am & lm;
+                amb[k] = am & ~(fm | lm);
                 any_amb |= amb[k];
-                if constexpr (CNT) nvisit += act ? 1 : 0;
-                if (act & far) {
+                if constexpr (CNT) nvisit += (resume[k] <= n) ? 1 : 0;
+                if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
                     resume[k] = t.escape;
                     if constexpr (CNT) ++nfar;
                     else far_set(k, nr, d, r2);
                 }
-                nearl[k] = act & !far & !amb[k];
-                any_near |= nearl[k];
             }
-            if (__builtin_expect(__ballot(any_amb) != 0, 0)) {
+            if (__builtin_expect(any_amb != 0, 0)) {
                 // the FP64 test with the upward pass's FP64 centre, then the
                 // same far / near work as above
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    if (amb[k]) {
+                    const bool ak = __builtin_amdgcn_inverse_ballot_w64(amb[k]);
+                    bool far64 = false;
+                    if (ak) {
                         const double* c64 = p.r64 + 3 * ((int64_t)k * p.nm + n);
-                        if (dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > t.thr2) {
-                            resume[k] = t.escape;
-                            if constexpr (CNT) {
-                                ++nfar;
-                            } else {
-                                float d[3];
-                                const float r2 = dist_set(nrs[k], d);
-                                far_set(k, nrs[k], d, r2);
-                            }
+                        far64 = dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > t.thr2;
+                    }
+                    if (ak & far64) {
+                        resume[k] = t.escape;
+                        if constexpr (CNT) {
+                            ++nfar;
                         } else {
-                            nearl[k] = true;
-                            any_near = true;
+                            float d[3];
+                            const float r2 = dist_set(nrs[k], d);
+                            far_set(k, nrs[k], d, r2);
                         }
                     }
+                    nm[k] |= __builtin_amdgcn_ballot_w64(ak & !far64);
                 }
             }
             if (topo_leaf(t)) break;
-            n = __ballot(any_near) == 0 ? t.escape : n + 1;
+            uint64_t nm_any = nm[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) nm_any |= nm[k];
+            n = nm_any == 0 ? t.escape : n + 1;
             if (n >= stop) break;
             }
+            bool nearl[K];
+            uint64_t nm_any = nm[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) nm_any |= nm[k];
+#pragma unroll
+            for (int k = 0; k < K; ++k) nearl[k] = __builtin_amdgcn_inverse_ballot_w64(nm[k]);
+            const bool any_near = __builtin_amdgcn_inverse_ballot_w64(nm_any);
             const bool leaf = topo_leaf(t);
             if (leaf) {
 #pragma unroll
@@ -655,13 +674,12 @@ __device__ __forceinline__ void trav_s(
                     if (nearl[k]) resume[k] = t.escape;
                 if constexpr (CNT) {
                     for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
-                } else if (__ballot(any_near) != 0) {
+                } else if (nm_any != 0) {
                     const bool sure = leaf_surely_outside(xf, p.lbound[n], wf);
                     const bool fast = __ballot(any_near && !sure) == 0;
-                    const uint64_t nm = __ballot(any_near);
-                    const int lr = fast ? spread_log2(__popcll(nm), t.end - t.begin) : 0;
+                    const int lr = fast ? spread_log2(__popcll(nm_any), t.end - t.begin) : 0;
                     if (lr > 0) {   // few near lanes: the idle lanes share the points
-                        const Spread sp = spread_setup(nm, any_near, lr);
+                        const Spread sp = spread_setup(nm_any, any_near, lr);
                         float mk[K], qx[3], ql[3] = {0.f, 0.f, 0.f}, part[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                         for (int k = 0; k < K; ++k)
