@@ -46,6 +46,8 @@ struct AP {
     int wdtype;
     double an, sign;
     double open_factor;       // 2 / theta
+    const double* c64;        // FP64 centres of the node records, [m, 3] (retests of the
+                              // FP32 opening and cutoff tests)
     double* adj_node;         // [m,3] accumulators L
     double* adj_pt;           // [3][n] accumulators N (sorted order, one plane per component,
                               // so a wave's atomics on one component touch consecutive words)
@@ -273,31 +275,43 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         const TopoRec t = rec32(topo, n);
         const NodeV32 nr = rec32(nodes, n);
         // the step's records loaded together and waited for once
-        CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
-                        "s"(nr.c[2]), "s"(nr.e[0]), "s"(nr.e[1]), "s"(nr.e[2]));
+        CAGR_LOADS_HERE("s"(t.escape), "s"(t.info), "s"(nr.cf[0]), "s"(nr.cf[1]), "s"(nr.cf[2]),
+                        "s"(nr.hi2), "s"(nr.lo2), "s"(nr.e[0]), "s"(nr.e[1]), "s"(nr.e[2]));
         __builtin_amdgcn_sched_barrier(0);
-        const bool act = resume <= n;
-        const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
-        const double d2 = dist2_rn(e0, e1, e2);
-        const bool far = d2 > t.thr2;
-        const bool takefar = act && far;
-        if (__ballot(takefar) != 0) {
+        // the forward walk's decisions: FP32 distance to the float centre
+        // against the record's bounds, the FP64 test in between
+        const float d[3] = {xf[0] - nr.cf[0], xf[1] - nr.cf[1], xf[2] - nr.cf[2]};
+        const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+        const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= n);
+        const uint64_t fm = __builtin_amdgcn_ballot_w64(r2 > nr.hi2);
+        const uint64_t lm = __builtin_amdgcn_ballot_w64(r2 < nr.lo2);
+        uint64_t far_m = am & fm, near_m = am & lm;
+        if (const uint64_t amb = am & ~(fm | lm); __builtin_expect(amb != 0, 0)) {
+            const bool ak = __builtin_amdgcn_inverse_ballot_w64(amb);
+            bool far64 = false;
+            if (ak) {
+                const double* c64 = p.c64 + 3 * (int64_t)n;
+                far64 = dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > t.thr2;
+            }
+            far_m |= __builtin_amdgcn_ballot_w64(ak & far64);
+            near_m |= __builtin_amdgcn_ballot_w64(ak & !far64);
+        }
+        const bool takefar = __builtin_amdgcn_inverse_ballot_w64(far_m);
+        if (far_m != 0) {
             resume = takefar ? t.escape : resume;
             if constexpr (FM == FM_COUNT) {
                 if (lane == 0) atomicAdd(p.node_cnt + n, 1);
             } else if constexpr (FM == FM_FILL) {
-                const uint64_t fmask = __ballot(takefar);
                 if (lane == 0 && k < kmax) {
                     p.slot_node[slot0 + k] = (uint64_t)n;
                     p.slot_id[slot0 + k] = (int32_t)(slot0 + k);
                     p.far_node[slot0 + k] = n;
-                    p.far_mask[slot0 + k] = fmask;
+                    p.far_mask[slot0 + k] = far_m;
                 }
             } else {
-                const float d[3] = {(float)e0, (float)e1, (float)e2};
-                const float r2 = (float)d2;
                 bool in = false;   // far at depth <= dsafe proves r > w
-                if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, takefar);
+                if (topo_depth(t) > dsafe)
+                    in = inside_centre(r2, w2, wf, nr.ce, x, p.c64 + 3 * (int64_t)n, w, takefar);
                 float z[3] = {0.f, 0.f, 0.f};
                 adj_term32<GA, CONT>(z, d, r2, takefar, in, h, nr.e, hanf, gin, winv, ew);
                 float s0, s1, s2;
@@ -310,9 +324,9 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             }
             ++k;
         }
-        const bool nearl = act && !far;
-        const uint64_t qmask = __ballot(nearl);
+        const uint64_t qmask = near_m;
         if (topo_leaf(t)) {
+            const bool nearl = __builtin_amdgcn_inverse_ballot_w64(near_m);
             resume = nearl ? t.escape : resume;
             if (MATH && qmask != 0)
                 adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
@@ -396,18 +410,17 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
         const int32_t n = __builtin_amdgcn_readlane(node_l, k & 63);
         const uint64_t m = lane_bcast_u64(mask_l, k & 63);
         const int32_t pos = __builtin_amdgcn_readlane(pos_l, k & 63);
-        const bool take = (m >> lane) & 1;
+        const bool take = __builtin_amdgcn_inverse_ballot_w64(m);
         const TopoRec t = rec32(topo, n);
         const NodeV32 nr = rec32(nodes, n);
-        CAGR_LOADS_HERE("s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]), "s"(nr.c[2]), "s"(nr.e[0]),
+        CAGR_LOADS_HERE("s"(t.info), "s"(nr.cf[0]), "s"(nr.cf[1]), "s"(nr.cf[2]), "s"(nr.e[0]),
                         "s"(nr.e[1]), "s"(nr.e[2]));
         __builtin_amdgcn_sched_barrier(0);
-        const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
-        const double d2 = dist2_rn(e0, e1, e2);
-        const float d[3] = {(float)e0, (float)e1, (float)e2};
-        const float r2 = (float)d2;
+        const float d[3] = {xf[0] - nr.cf[0], xf[1] - nr.cf[1], xf[2] - nr.cf[2]};
+        const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
         bool in = false;   // far at depth <= dsafe proves r > w
-        if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w, take);
+        if (topo_depth(t) > dsafe)
+            in = inside_centre(r2, w2, wf, nr.ce, x, p.c64 + 3 * (int64_t)n, w, take);
         float z[3] = {0.f, 0.f, 0.f};
         adj_term32<GA, CONT>(z, d, r2, take, in, h, nr.e, hanf, gin, winv, ew);
         float s0, s1, s2;
@@ -620,10 +633,11 @@ __global__ __launch_bounds__(kThreads) void k_adj_down(int64_t n, const int32_t*
 
 int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
             int continuous, double an, void* out, hipStream_t st, bool planned,
-            const void* nodes_in, const void* src_in) {
+            const void* nodes_in, const void* src_in, const double* c64_in) {
     if (!nodes_in) nodes_in = t->nodes;
     if (!src_in) src_in = t->src;
     AP p{};
+    p.c64 = c64_in ? c64_in : t->r64;
     p.n = t->n;
     p.root = 0;
     p.stop = (int32_t)t->m;
@@ -722,7 +736,9 @@ void adjoint_plan_free(cagr_tree* t) {
     (void)hipFree(t->plan_near_mask);
     (void)hipFree(t->plan_nodes);
     (void)hipFree(t->plan_src);
+    (void)hipFree(t->plan_c64);
     t->plan_nodes = t->plan_src = nullptr;
+    t->plan_c64 = nullptr;
     t->plan_rec_an = __builtin_nan("");
     t->plan_w = nullptr;
     t->plan_wave_off = t->plan_node_off = t->plan_slot_pos = nullptr;
@@ -761,6 +777,7 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
     p.wave_cnt = t->plan_wave_off;
     p.near_cnt = t->plan_near_off;
     p.node_cnt = t->plan_node_off;
+    p.c64 = t->r64;   // the centres of the upward pass above
     auto nodes = (const NodeV32*)t->nodes;
     auto src = (const SrcV32*)t->src;
     p.tile_order = t->tile_order;

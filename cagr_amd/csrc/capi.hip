@@ -481,22 +481,28 @@ int cagr_apply_adjoint_planned(cagr_tree* t, int op, int precision, const void* 
         if (rc) return rc;
         if (keep) {
             const size_t nb = (size_t)t->m * sizeof(NodeV32), sb = (size_t)t->n * sizeof(SrcV32);
-            if (!t->plan_nodes || !t->plan_src) {
+            const size_t cb = (size_t)t->m * 3 * sizeof(double);
+            if (!t->plan_nodes || !t->plan_src || !t->plan_c64) {
                 (void)hipFree(t->plan_nodes);
                 (void)hipFree(t->plan_src);
+                (void)hipFree(t->plan_c64);
                 t->plan_nodes = t->plan_src = nullptr;
+                t->plan_c64 = nullptr;
                 CAGR_HIP(hipMalloc(&t->plan_nodes, nb));
                 CAGR_HIP(hipMalloc(&t->plan_src, sb));
+                CAGR_HIP(hipMalloc(&t->plan_c64, cb));
             }
             CAGR_HIP(hipMemcpyAsync(t->plan_nodes, t->nodes, nb, hipMemcpyDeviceToDevice, st));
             CAGR_HIP(hipMemcpyAsync(t->plan_src, t->src, sb, hipMemcpyDeviceToDevice, st));
+            CAGR_HIP(hipMemcpyAsync(t->plan_c64, t->r64, cb, hipMemcpyDeviceToDevice, st));
             t->plan_rec_an = an_rec;
         }
     }
     if ((rc = prof_mark(t, 1, st))) return rc;
     if ((rc = prof_mark(t, 0, st))) return rc;
     rc = adjoint(t, op, precision, g, dtype, widths, continuous, an, out, st, true,
-                 keep ? t->plan_nodes : nullptr, keep ? t->plan_src : nullptr);
+                 keep ? t->plan_nodes : nullptr, keep ? t->plan_src : nullptr,
+                 keep ? t->plan_c64 : nullptr);
     if (rc) return rc;
     return prof_mark(t, 0, st);
 }

@@ -46,8 +46,10 @@ struct alignas(16) SrcV64 { double y[3]; double mu[3]; };
 struct alignas(16) SrcV32 { float y[3]; float mu[3]; float e[3]; float q[3]; };
 //   e[k] = exp(-An*y_k/2), q[k] = An/2 * e[k] * mu[k]
 struct alignas(16) NodeV64 { double c[3]; double m[3]; };
-struct alignas(16) NodeV32 { double c[3]; float m[3]; float e[3]; float q[3]; float pad; };
-//   c = weighted centre (FP64, decides far/near), m = attr sum, e/q as above at c
+struct alignas(16) NodeV32 { float cf[3]; float hi2, lo2, ce; float m[3]; float e[3]; float q[3]; float pad; };
+//   cf = (float)c of the weighted centre c, hi2 / lo2 / ce its opening and
+//   cutoff bounds (as in NodeS32; c itself stays in the upward pass's r64),
+//   m = attr sum, e/q as above at c
 
 // scalar family (attr_dim 1 per set: AT, A2Tk, fused sum_k A2Tk)
 template <int K> struct alignas(16) SrcS64 { double y[3]; double s[K]; };
@@ -141,9 +143,11 @@ struct cagr_tree {
     int64_t plan_slots = -1;           // -1: no plan
     // FP32 node and source records of the plan's centres (an upward pass of
     // plan_w with no attributes), kept after the first planned call with
-    // convection factor plan_rec_an (NaN: none kept)
+    // convection factor plan_rec_an (NaN: none kept), and those FP64 centres
+    // ([m, 3], the cutoff tests' retests)
     void* plan_nodes = nullptr;
     void* plan_src = nullptr;
+    double* plan_c64 = nullptr;
     double plan_rec_an = __builtin_nan("");
     int persistent = 0;              // 1: traversal through the item queue
     int waves_per_block = 4;         // launch shape of the persistent traversal kernels
@@ -217,10 +221,12 @@ struct Queries {
 // implemented in adjoint.hip (needs the node records of an upward pass with
 // the same fixed weights)
 // (planned: FP32 far steps go through the tree's adjoint plan instead of atomics;
-// nodes / src: records to read instead of the tree's own, when not null)
+// nodes / src / c64: records and their FP64 centres to read instead of the
+// tree's own, when not null)
 int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const void* widths,
             int continuous, double an, void* out, hipStream_t st, bool planned = false,
-            const void* nodes = nullptr, const void* src = nullptr);
+            const void* nodes = nullptr, const void* src = nullptr,
+            const double* c64 = nullptr);
 // builds the adjoint plan for point weights (original order; centres of an
 // upward pass with them)
 int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st);

@@ -46,13 +46,13 @@ __device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, d
 
 // r >= w for a far node decided as the FP64 reference would, from the FP32
 // distance^2 to the node's float centre: r2 carries the centre's rounding
-// (ce per component, NodeV32) besides its own, so the band where FP32 cannot
-// call it is widened by 4 w ce + 4 ce^2; lanes in the band retest with the
-// FP64 centre c (read only then)
+// (ce per component, NodeV32 / NodeS32) besides its own, so the band where
+// FP32 cannot call it is widened by 4 w ce + 4 ce^2; lanes in the band retest
+// with the FP64 centre c (read only then; lanes with use = false never do)
 __device__ inline bool inside_centre(float r2, float w2, float wf, float ce, const double* x,
-                                     const double* c, double w) {
+                                     const double* c, double w, bool use = true) {
     const float band = fmaf(4.f * wf, ce, fmaf(4.f * ce, ce, 1e-5f * w2));
-    const bool amb = fabsf(r2 - w2) <= band;
+    const bool amb = use && fabsf(r2 - w2) <= band;
     bool in = r2 < w2;
     if (__builtin_expect(__ballot(amb) != 0, 0)) {
         if (amb) in = !(sqrt(dist2_rn(x[0] - c[0], x[1] - c[1], x[2] - c[2])) >= w);
@@ -284,12 +284,45 @@ __device__ inline Spread spread_setup(uint64_t nm, bool nearl, int lr) {
     return s;
 }
 
-// sum over the 2^lr lanes of a slot (every lane of the slot gets the same
-// total: each butterfly step adds the same two values in either lane)
+// sum over the 2^lr lanes of a slot, lr <= 4 (every lane of the slot gets the
+// same total: each step adds the same two values in either lane).  The steps
+// are DPP lane moves inside rows of 16, each one vector add with a moved
+// operand: lanes l and l^1, l and l^2 (quad permutations), then the two
+// quads of a half row (half-row mirror) and the two halves of a row (row
+// mirror) -- a shuffle through LDS would cost an address computation, a
+// ds_bpermute and its wait per step
+template <int CTRL>
+__device__ inline double dpp_move_f64(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf,
+                                                              false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int CTRL>
+__device__ inline float dpp_step(float v) { return dpp_move<CTRL, 0xf>(v); }
+template <int CTRL>
+__device__ inline double dpp_step(double v) { return dpp_move_f64<CTRL>(v); }
+
 template <typename T>
 __device__ inline T spread_sum(T v, int lr) {
-    for (int s = 1; s < (1 << lr); s <<= 1) v += __shfl_xor(v, s, 64);
+    if (lr >= 1) v += dpp_step<0xb1>(v);    // quad_perm [1,0,3,2]
+    if (lr >= 2) v += dpp_step<0x4e>(v);    // quad_perm [2,3,0,1]
+    if (lr >= 3) v += dpp_step<0x141>(v);   // row_half_mirror
+    if (lr >= 4) v += dpp_step<0x140>(v);   // row_mirror
     return v;
+}
+// the same for NV values at once (one branch per step for all of them)
+template <int NV, typename T>
+__device__ inline void spread_sums(T* v, int lr) {
+    if (lr >= 1)
+        for (int i = 0; i < NV; ++i) v[i] += dpp_step<0xb1>(v[i]);
+    if (lr >= 2)
+        for (int i = 0; i < NV; ++i) v[i] += dpp_step<0x4e>(v[i]);
+    if (lr >= 3)
+        for (int i = 0; i < NV; ++i) v[i] += dpp_step<0x141>(v[i]);
+    if (lr >= 4)
+        for (int i = 0; i < NV; ++i) v[i] += dpp_step<0x140>(v[i]);
 }
 
 // lane `lane`'s 64-bit value to every lane (a scalar)

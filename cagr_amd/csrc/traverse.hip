@@ -276,36 +276,53 @@ __device__ __forceinline__ void trav_v(
             const NodeV32 nr = rec32(nodes, n);
             // the step's records loaded together and waited for once (the far
             // interaction's fields would otherwise be a second round trip)
-            CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
-                            "s"(nr.c[2]), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]), "s"(nr.e[0]),
-                            "s"(nr.e[1]), "s"(nr.e[2]), "s"(nr.q[0]), "s"(nr.q[1]), "s"(nr.q[2]));
+            CAGR_LOADS_HERE("s"(t.escape), "s"(t.info), "s"(nr.cf[0]), "s"(nr.cf[1]), "s"(nr.cf[2]),
+                            "s"(nr.hi2), "s"(nr.lo2), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]),
+                            "s"(nr.e[0]), "s"(nr.e[1]), "s"(nr.e[2]), "s"(nr.q[0]), "s"(nr.q[1]),
+                            "s"(nr.q[2]));
             __builtin_amdgcn_sched_barrier(0);
-            const double e0 = x[0] - nr.c[0], e1 = x[1] - nr.c[1], e2 = x[2] - nr.c[2];
-            const double d2 = dist2_rn(e0, e1, e2);
-            // each comparison as its own wave mask, combined on the scalar
-            // unit (a ballot of a combined lane predicate, or a predicate
-            // carried past the far branch, would go through a vector register)
+            // FP32 distance to the float centre: far above the record's hi2,
+            // near below lo2; lanes between them take the FP64 test with the
+            // upward pass's FP64 centre (open_band), so every decision is the
+            // FP64 test's.  Each comparison is its own wave mask, combined on
+            // the scalar unit (a ballot of a combined lane predicate, or a
+            // predicate carried past the far branch, would go through a vector
+            // register).
+            float d[3] = {xf[0] - nr.cf[0], xf[1] - nr.cf[1], xf[2] - nr.cf[2]};
+            if constexpr (QM != QM_SELF)
+                for (int k = 0; k < 3; ++k) d[k] += xl[k];
+            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
             const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= n);
-            const uint64_t fm = __builtin_amdgcn_ballot_w64(d2 > t.thr2);
-            const uint64_t near_m = am & ~fm;
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(r2 > nr.hi2);
+            const uint64_t lm = __builtin_amdgcn_ballot_w64(r2 < nr.lo2);
+            uint64_t far_m = am & fm, near_m = am & lm;
+            if (const uint64_t amb = am & ~(fm | lm); __builtin_expect(amb != 0, 0)) {
+                const bool ak = __builtin_amdgcn_inverse_ballot_w64(amb);
+                bool far64 = false;
+                if (ak) {
+                    const double* c64 = p.r64 + 3 * (int64_t)n;
+                    far64 = dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > t.thr2;
+                }
+                far_m |= __builtin_amdgcn_ballot_w64(ak & far64);
+                near_m |= __builtin_amdgcn_ballot_w64(ak & !far64);
+            }
             if constexpr (CNT) nvisit += (resume <= n) ? 1 : 0;
-            if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
+            if (__builtin_amdgcn_inverse_ballot_w64(far_m)) {
                 resume = t.escape;
                 if constexpr (CNT) {
                     ++nfar;
                 } else {
-                    const float d[3] = {(float)e0, (float)e1, (float)e2};
-                    const float r2 = (float)d2;
                     // far at depth <= dsafe proves r > w: the cutoff test (and
                     // its select) only below that depth
+                    const double* c64 = p.r64 + 3 * (int64_t)n;
                     if constexpr (CONT) {
                         bool in = false;
-                        if (topo_depth(t) > dsafe) in = inside_fp32(r2, w2, e0, e1, e2, w);
+                        if (topo_depth(t) > dsafe) in = inside_centre(r2, w2, wf, nr.ce, x, c64, w);
                         acc.add(acc.fp, d, r2, in, nr.m, nr.e, nr.q);
                     } else {
                         float ir = __builtin_amdgcn_rsqf(r2);
                         if (topo_depth(t) > dsafe) {
-                            if (inside_fp32(r2, w2, e0, e1, e2, w)) ir = 0.f;
+                            if (inside_centre(r2, w2, wf, nr.ce, x, c64, w)) ir = 0.f;
                         }
                         acc.add_ir(acc.fp, d, ir, nr.m, nr.e, nr.q);
                     }
@@ -338,8 +355,9 @@ __device__ __forceinline__ void trav_v(
                                 acc.add(part, d, r2, false, sr.mu, sr.e, sr.q);
                             }
                         }
+                        spread_sums<acc.NO>(part, lr);
                         for (int k = 0; k < acc.NO; ++k) {
-                            const float v = lane_pull(spread_sum(part[k], lr), sp.rank << lr);
+                            const float v = lane_pull(part[k], sp.rank << lr);
                             if (nearl) acc.lp[k] += v;
                         }
                         if (nearl) acc.flush_leaf();
@@ -389,22 +407,24 @@ __device__ __forceinline__ void trav_v(
             CAGR_LOADS_HERE("s"(t.thr2), "s"(t.escape), "s"(t.info), "s"(nr.c[0]), "s"(nr.c[1]),
                             "s"(nr.c[2]), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]));
             __builtin_amdgcn_sched_barrier(0);
-            const bool act = resume <= n;
             const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
-            const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
-            if constexpr (CNT) nvisit += act ? 1 : 0;
-            if (act && far) {
+            // comparisons as wave masks, as in the FP32c walk
+            const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= n);
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(dist2_rn(d[0], d[1], d[2]) > t.thr2);
+            const uint64_t near_m = am & ~fm;
+            if constexpr (CNT) nvisit += (resume <= n) ? 1 : 0;
+            if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
                 resume = t.escape;
                 if constexpr (CNT) ++nfar;
                 else acc.add(d, w, nr.m, p.an);
             }
-            const bool nearl = act && !far;
             const bool leaf = topo_leaf(t);
             if (leaf) {
+                const bool nearl = __builtin_amdgcn_inverse_ballot_w64(near_m);
                 if (nearl) resume = t.escape;
                 if constexpr (CNT) {
                     if (nearl) nnear += t.end - t.begin;
-                } else if (const uint64_t nm = __ballot(nearl); nm != 0) {
+                } else if (const uint64_t nm = near_m; nm != 0) {
                     const int lr = spread_log2(__popcll(nm), t.end - t.begin);
                     if (lr > 0) {   // few near lanes: the idle lanes share the points
                         const Spread sp = spread_setup(nm, nearl, lr);
@@ -444,7 +464,7 @@ __device__ __forceinline__ void trav_v(
             }
             // one latch: past a leaf or a subtree no lane enters, the escape;
             // else the first child
-            n = (leaf || __ballot(nearl) == 0) ? t.escape : n + 1;
+            n = (leaf || near_m == 0) ? t.escape : n + 1;
         }
         if constexpr (!CNT) {
             if (valid) {
@@ -697,8 +717,9 @@ __device__ __forceinline__ void trav_s(
                                 near_pair(part, sr, d, r2, false, mk);
                             }
                         }
+                        spread_sums<3>(part, lr);
                         for (int cc = 0; cc < 3; ++cc) {
-                            const float v = lane_pull(spread_sum(part[cc], lr), sp.rank << lr);
+                            const float v = lane_pull(part[cc], sp.rank << lr);
                             if (any_near) lp[cc] += v;
                         }
                         if (any_near)
@@ -769,8 +790,7 @@ __device__ __forceinline__ void trav_s(
             // internal nodes until a leaf or the end of the tree, then the leaf
             // (see the FP32c walk)
             TopoRec t;
-            bool nearl[K];
-            bool any_near;
+            uint64_t nm[K];   // near lanes of each set (wave masks, as in the FP32c walk)
             for (;;) {
             n = __builtin_amdgcn_readfirstlane(n);
             t = rec32(topo, n);
@@ -788,36 +808,45 @@ __device__ __forceinline__ void trav_s(
                                 "s"(nrs[0].c[0]), "s"(nrs[0].c[1]), "s"(nrs[0].c[2]), "s"(nrs[0].s));
             }
             __builtin_amdgcn_sched_barrier(0);
-            any_near = false;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const NodeS64& nr = nrs[k];
-                const bool act = resume[k] <= n;
                 const double d[3] = {x[0] - nr.c[0], x[1] - nr.c[1], x[2] - nr.c[2]};
-                const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
-                if constexpr (CNT) nvisit += act ? 1 : 0;
-                if (act && far) {
+                const uint64_t am = __builtin_amdgcn_ballot_w64(resume[k] <= n);
+                const uint64_t fm =
+                    __builtin_amdgcn_ballot_w64(dist2_rn(d[0], d[1], d[2]) > t.thr2);
+                nm[k] = am & ~fm;
+                if constexpr (CNT) nvisit += (resume[k] <= n) ? 1 : 0;
+                if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
                     resume[k] = t.escape;
                     if constexpr (CNT) ++nfar;
                     else add(d, nr.s, p.kdir + k);
                 }
-                nearl[k] = act && !far;
-                any_near |= nearl[k];
             }
             if (topo_leaf(t)) break;
-            n = __ballot(any_near) == 0 ? t.escape : n + 1;
+            uint64_t nm_any = nm[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) nm_any |= nm[k];
+            n = nm_any == 0 ? t.escape : n + 1;
             if (n >= stop) break;
             }
+            bool nearl[K];
+            uint64_t nm_any = nm[0];
+#pragma unroll
+            for (int k = 1; k < K; ++k) nm_any |= nm[k];
+#pragma unroll
+            for (int k = 0; k < K; ++k) nearl[k] = __builtin_amdgcn_inverse_ballot_w64(nm[k]);
+            const bool any_near = __builtin_amdgcn_inverse_ballot_w64(nm_any);
             const bool leaf = topo_leaf(t);
             if (leaf) {
                 for (int k = 0; k < K; ++k)
                     if (nearl[k]) resume[k] = t.escape;
                 if constexpr (CNT) {
                     for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
-                } else if (const uint64_t nm = __ballot(any_near); nm != 0) {
-                    const int lr = spread_log2(__popcll(nm), t.end - t.begin);
+                } else if (nm_any != 0) {
+                    const int lr = spread_log2(__popcll(nm_any), t.end - t.begin);
                     if (lr > 0) {   // few near lanes: the idle lanes share the points
-                        const Spread sp = spread_setup(nm, any_near, lr);
+                        const Spread sp = spread_setup(nm_any, any_near, lr);
                         int bits = 0;
                         for (int k = 0; k < K; ++k) bits |= nearl[k] ? 1 << k : 0;
                         const int qbits = lane_pull(bits, sp.owner);
@@ -833,8 +862,9 @@ __device__ __forceinline__ void trav_s(
                                     if (qbits & (1 << k)) add_to(part, d, sr.s[k], p.kdir + k, qw);
                             }
                         }
+                        spread_sums<3>(part, lr);
                         for (int cc = 0; cc < 3; ++cc) {
-                            const double v = lane_pull(spread_sum(part[cc], lr), sp.rank << lr);
+                            const double v = lane_pull(part[cc], sp.rank << lr);
                             if (any_near) acc[cc] += v;
                         }
                     } else if (any_near) {

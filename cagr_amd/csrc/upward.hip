@@ -90,16 +90,16 @@ template <int P>
 struct Rec<0, 1, P> {
     static constexpr int C = 3;
     __device__ static void node(void* nodes, int64_t n, int k, const double* c, const double* mm,
-                                double an, int /*dir0*/, double /*thr2*/) {
+                                double an, int /*dir0*/, double thr2) {
         if constexpr (P == CAGR_PREC_FP64) {
             NodeV64 r;
             for (int d = 0; d < 3; ++d) { r.c[d] = c[d]; r.m[d] = mm[d]; }
             ((NodeV64*)nodes)[n] = r;
         } else {
             NodeV32 r;
+            open_band(c, thr2, r.cf, r.hi2, r.lo2, r.ce);
             for (int d = 0; d < 3; ++d) {
                 const double e = exp(-0.5 * an * c[d]);
-                r.c[d] = c[d];
                 r.m[d] = (float)mm[d];
                 r.e[d] = (float)e;
                 r.q[d] = (float)(0.5 * an * e * mm[d]);
