@@ -298,7 +298,6 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         }
         const bool takefar = __builtin_amdgcn_inverse_ballot_w64(far_m);
         if (far_m != 0) {
-            resume = takefar ? t.escape : resume;
             if constexpr (FM == FM_COUNT) {
                 if (lane == 0) atomicAdd(p.node_cnt + n, 1);
             } else if constexpr (FM == FM_FILL) {
@@ -325,9 +324,10 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
             ++k;
         }
         const uint64_t qmask = near_m;
+        // past a leaf or a subtree no lane enters, the escape (every lane
+        // active at n stays active there); into the first child with the far
+        // lanes parked past the subtree
         if (topo_leaf(t)) {
-            const bool nearl = __builtin_amdgcn_inverse_ballot_w64(near_m);
-            resume = nearl ? t.escape : resume;
             if (MATH && qmask != 0)
                 adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
             if constexpr (FM == FM_COUNT || FM == FM_FILL) {
@@ -342,8 +342,11 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
                 }
             }
             n = t.escape;
+        } else if (qmask != 0) {
+            if (takefar) resume = t.escape;
+            n = n + 1;
         } else {
-            n = qmask != 0 ? n + 1 : t.escape;
+            n = t.escape;
         }
     }
     if constexpr (FM == FM_COUNT)
@@ -553,10 +556,7 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
         const bool far = dist2_rn(d[0], d[1], d[2]) > t.thr2;
         const bool takefar = act && far;
         double z[3] = {0.0, 0.0, 0.0};
-        if (takefar) {
-            resume = t.escape;
-            adj_term64<GA, CONT>(z, d, w, gk, p.an, p.sign);
-        }
+        if (takefar) adj_term64<GA, CONT>(z, d, w, gk, p.an, p.sign);
         if (__ballot(takefar) != 0) {
             double s0, s1, s2;
             wave_sum3(z[0], z[1], z[2], s0, s1, s2);
@@ -567,8 +567,8 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
             }
         }
         const bool nearl = act && !far;
+        // resume moves only into a subtree (as in the FP32c walk)
         if (topo_leaf(t)) {
-            if (nearl) resume = t.escape;
             const uint64_t qmask = __ballot(nearl);
             if (qmask != 0) {   // transposed, in the group layout of near_groups
                 const NearGroups ng = near_groups(qmask, t.end - t.begin, lane);
@@ -601,8 +601,11 @@ __global__ __launch_bounds__(kThreads) void k_adj64(AP p, const TopoRec* __restr
                 }
             }
             n = t.escape;
+        } else if (__ballot(nearl) != 0) {
+            if (takefar) resume = t.escape;
+            n = n + 1;
         } else {
-            n = __ballot(nearl) != 0 ? n + 1 : t.escape;
+            n = t.escape;
         }
     }
 }

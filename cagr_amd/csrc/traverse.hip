@@ -14,12 +14,14 @@
 // is a wave-uniform (scalar) load.  Each lane keeps `resume`, the first node
 // id it still has to consider: a lane is active at n iff resume <= n.  An
 // active lane that finds n far takes the far interaction and skips n's
-// subtree (resume = escape(n)); at a near leaf it takes the leaf's points and
-// also skips past it; at a near internal node it stays active for the
-// children.  The wave moves to n+1 if any active lane opened n, else to
-// escape(n).  Every lane therefore visits exactly the nodes of its own
-// reference DFS and takes exactly the same far/near decisions; only the order
-// of the summands differs.
+// subtree; at a near leaf it takes the leaf's points and also skips past it;
+// at a near internal node it stays active for the children.  The wave moves
+// to n+1 if any active lane opened n, else to escape(n).  Only the move to
+// n+1 needs the far lanes' resume = escape(n): a move to escape(n) leaves
+// every lane active at n active there anyway, so leaves and subtrees no lane
+// enters update nothing.  Every lane therefore visits exactly the nodes of
+// its own reference DFS and takes exactly the same far/near decisions; only
+// the order of the summands differs.
 //
 // Far/near tests run in FP64 in both precisions (the decision must not
 // depend on the precision of the kernel sums); in the FP32 form a cutoff
@@ -308,7 +310,6 @@ __device__ __forceinline__ void trav_v(
             }
             if constexpr (CNT) nvisit += (resume <= n) ? 1 : 0;
             if (__builtin_amdgcn_inverse_ballot_w64(far_m)) {
-                resume = t.escape;
                 if constexpr (CNT) {
                     ++nfar;
                 } else {
@@ -331,7 +332,6 @@ __device__ __forceinline__ void trav_v(
             const bool leaf = topo_leaf(t);
             if (leaf) {
                 const bool nearl = __builtin_amdgcn_inverse_ballot_w64(near_m);
-                if (nearl) resume = t.escape;
                 if constexpr (CNT) {
                     if (nearl) nnear += t.end - t.begin;
                 } else if (near_m != 0) {
@@ -387,8 +387,13 @@ __device__ __forceinline__ void trav_v(
                 }
             }
             // one latch: past a leaf or a subtree no lane enters, the escape;
-            // else the first child
-            n = (leaf || near_m == 0) ? t.escape : n + 1;
+            // else the first child, with the far lanes parked past the subtree
+            if (leaf || near_m == 0) {
+                n = t.escape;
+            } else {
+                if (__builtin_amdgcn_inverse_ballot_w64(far_m)) resume = t.escape;
+                n = n + 1;
+            }
         }
         if constexpr (!CNT) {
             acc.flush_leaf();
@@ -411,17 +416,15 @@ __device__ __forceinline__ void trav_v(
             // comparisons as wave masks, as in the FP32c walk
             const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= n);
             const uint64_t fm = __builtin_amdgcn_ballot_w64(dist2_rn(d[0], d[1], d[2]) > t.thr2);
-            const uint64_t near_m = am & ~fm;
+            const uint64_t near_m = am & ~fm, far_m = am & fm;
             if constexpr (CNT) nvisit += (resume <= n) ? 1 : 0;
-            if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
-                resume = t.escape;
+            if (__builtin_amdgcn_inverse_ballot_w64(far_m)) {
                 if constexpr (CNT) ++nfar;
                 else acc.add(d, w, nr.m, p.an);
             }
             const bool leaf = topo_leaf(t);
             if (leaf) {
                 const bool nearl = __builtin_amdgcn_inverse_ballot_w64(near_m);
-                if (nearl) resume = t.escape;
                 if constexpr (CNT) {
                     if (nearl) nnear += t.end - t.begin;
                 } else if (const uint64_t nm = near_m; nm != 0) {
@@ -463,8 +466,13 @@ __device__ __forceinline__ void trav_v(
                 }
             }
             // one latch: past a leaf or a subtree no lane enters, the escape;
-            // else the first child
-            n = (leaf || near_m == 0) ? t.escape : n + 1;
+            // else the first child, with the far lanes parked past the subtree
+            if (leaf || near_m == 0) {
+                n = t.escape;
+            } else {
+                if (__builtin_amdgcn_inverse_ballot_w64(far_m)) resume = t.escape;
+                n = n + 1;
+            }
         }
         if constexpr (!CNT) {
             if (valid) {
@@ -623,7 +631,7 @@ __device__ __forceinline__ void trav_s(
                     for (int cc = 0; cc < 3; ++cc) d[cc] += xl[cc];
                 return fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
             };
-            uint64_t amb[K];
+            uint64_t amb[K], fmk[K];
             uint64_t any_amb = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -640,11 +648,11 @@ __device__ __forceinline__ void trav_s(
                 const uint64_t fm = __builtin_amdgcn_ballot_w64(r2 > nr.hi2);
                 const uint64_t lm = __builtin_amdgcn_ballot_w64(r2 < nr.lo2);
                 nm[k] = am & lm;
+                fmk[k] = am & fm;
                 amb[k] = am & ~(fm | lm);
                 any_amb |= amb[k];
                 if constexpr (CNT) nvisit += (resume[k] <= n) ? 1 : 0;
-                if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
-                    resume[k] = t.escape;
+                if (__builtin_amdgcn_inverse_ballot_w64(fmk[k])) {
                     if constexpr (CNT) ++nfar;
                     else far_set(k, nr, d, r2);
                 }
@@ -661,7 +669,6 @@ __device__ __forceinline__ void trav_s(
                         far64 = dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > t.thr2;
                     }
                     if (ak & far64) {
-                        resume[k] = t.escape;
                         if constexpr (CNT) {
                             ++nfar;
                         } else {
@@ -670,6 +677,7 @@ __device__ __forceinline__ void trav_s(
                             far_set(k, nrs[k], d, r2);
                         }
                     }
+                    fmk[k] |= __builtin_amdgcn_ballot_w64(ak & far64);
                     nm[k] |= __builtin_amdgcn_ballot_w64(ak & !far64);
                 }
             }
@@ -677,7 +685,16 @@ __device__ __forceinline__ void trav_s(
             uint64_t nm_any = nm[0];
 #pragma unroll
             for (int k = 1; k < K; ++k) nm_any |= nm[k];
-            n = nm_any == 0 ? t.escape : n + 1;
+            // into the first child with each set's far lanes parked past the
+            // subtree, or past the subtree (nothing to update)
+            if (nm_any == 0) {
+                n = t.escape;
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (__builtin_amdgcn_inverse_ballot_w64(fmk[k])) resume[k] = t.escape;
+                n = n + 1;
+            }
             if (n >= stop) break;
             }
             bool nearl[K];
@@ -689,9 +706,6 @@ __device__ __forceinline__ void trav_s(
             const bool any_near = __builtin_amdgcn_inverse_ballot_w64(nm_any);
             const bool leaf = topo_leaf(t);
             if (leaf) {
-#pragma unroll
-                for (int k = 0; k < K; ++k)
-                    if (nearl[k]) resume[k] = t.escape;
                 if constexpr (CNT) {
                     for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
                 } else if (nm_any != 0) {
@@ -790,7 +804,7 @@ __device__ __forceinline__ void trav_s(
             // internal nodes until a leaf or the end of the tree, then the leaf
             // (see the FP32c walk)
             TopoRec t;
-            uint64_t nm[K];   // near lanes of each set (wave masks, as in the FP32c walk)
+            uint64_t nm[K], fmk[K];   // near / far lanes of each set (wave masks, as in the FP32c walk)
             for (;;) {
             n = __builtin_amdgcn_readfirstlane(n);
             t = rec32(topo, n);
@@ -816,9 +830,9 @@ __device__ __forceinline__ void trav_s(
                 const uint64_t fm =
                     __builtin_amdgcn_ballot_w64(dist2_rn(d[0], d[1], d[2]) > t.thr2);
                 nm[k] = am & ~fm;
+                fmk[k] = am & fm;
                 if constexpr (CNT) nvisit += (resume[k] <= n) ? 1 : 0;
-                if (__builtin_amdgcn_inverse_ballot_w64(am & fm)) {
-                    resume[k] = t.escape;
+                if (__builtin_amdgcn_inverse_ballot_w64(fmk[k])) {
                     if constexpr (CNT) ++nfar;
                     else add(d, nr.s, p.kdir + k);
                 }
@@ -827,7 +841,14 @@ __device__ __forceinline__ void trav_s(
             uint64_t nm_any = nm[0];
 #pragma unroll
             for (int k = 1; k < K; ++k) nm_any |= nm[k];
-            n = nm_any == 0 ? t.escape : n + 1;
+            if (nm_any == 0) {
+                n = t.escape;
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (__builtin_amdgcn_inverse_ballot_w64(fmk[k])) resume[k] = t.escape;
+                n = n + 1;
+            }
             if (n >= stop) break;
             }
             bool nearl[K];
@@ -839,8 +860,6 @@ __device__ __forceinline__ void trav_s(
             const bool any_near = __builtin_amdgcn_inverse_ballot_w64(nm_any);
             const bool leaf = topo_leaf(t);
             if (leaf) {
-                for (int k = 0; k < K; ++k)
-                    if (nearl[k]) resume[k] = t.escape;
                 if constexpr (CNT) {
                     for (int k = 0; k < K; ++k) nnear += nearl[k] ? t.end - t.begin : 0;
                 } else if (nm_any != 0) {
