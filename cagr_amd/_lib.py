@@ -33,7 +33,8 @@ EXPORTS = (
     "cagr_apply_weighted", "cagr_apply_adjoint", "cagr_apply_queries",
     "cagr_apply_queries_noalloc", "cagr_apply_moments", "cagr_grid_query",
     "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
-    "cagr_order_items", "cagr_profile", "cagr_profile_read", "cagr_adjoint_plan",
+    "cagr_order_items", "cagr_split_walks", "cagr_profile", "cagr_profile_read",
+    "cagr_adjoint_plan",
     "cagr_apply_adjoint_planned", "cagr_pgr_build", "cagr_pgr_destroy", "cagr_pgr_info",
     "cagr_pgr_read", "cagr_pgr_box", "cagr_pgr_mesh", "cagr_pgr_mesh_read",
 )
@@ -60,6 +61,7 @@ _SIGS = {
     "cagr_forest_roots": (_C, [_P, _P]),
     "cagr_set_launch": (_C, [_P, _C, _C, _I]),
     "cagr_order_items": (_C, [_P, _P]),
+    "cagr_split_walks": (_C, [_P, _D, _C, _P, _P]),
     "cagr_profile": (_C, [_P, _C]),
     "cagr_profile_read": (_C, [_P, _C, _P, _P, _P, _I]),
     "cagr_adjoint_plan": (_C, [_P, _P, _C, _P]),

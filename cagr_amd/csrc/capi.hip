@@ -240,8 +240,25 @@ int cagr_set_launch(cagr_tree* t, int persistent, int waves_per_block, int64_t g
     return CAGR_OK;
 }
 
+int cagr_split_walks(cagr_tree* t, double piece_cost, int max_pieces, int64_t* counts,
+                     void* stream) {
+    CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    const hipStream_t st = (hipStream_t)stream;
+    int rc = call_begin(t, st);
+    if (rc) return rc;
+    CallEnd end_{t, st};
+    rc = split_walks(t, piece_cost, max_pieces, st);
+    if (rc) split_walks_free(t);
+    if (counts) {
+        counts[0] = t->num_pieces;
+        counts[1] = t->num_split;
+    }
+    return rc;
+}
+
 int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
     CAGR_REQUIRE(t != nullptr, CAGR_ERR_ARG, "null tree handle");
+    if (!query_cost) split_walks_free(t);
     if (t->num_items == 0) return CAGR_OK;
     if (!query_cost) {   // back to natural order, equal contiguous chunks
         (void)hipFree(t->item_order);
@@ -306,6 +323,7 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
 int cagr_tree_destroy(cagr_tree* t) {
     if (!t) return CAGR_OK;
     adjoint_plan_free(t);
+    split_walks_free(t);
     for (auto& ev : t->prof_events)
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     (void)hipFree(t->pts64);

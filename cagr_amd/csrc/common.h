@@ -155,6 +155,17 @@ struct cagr_tree {
     // dispatch order of the regular grid's waves (one wave per block; null:
     // Morton order): each XCD's contiguous range of waves, longest first
     int32_t* tile_order = nullptr;
+    // walk pieces of the self-query walks (cagr_split_walks; null: none): a
+    // wave whose walk costs more than the target is cut into node ranges,
+    // each run as its own wave {wave, first node, end node, partial slot}
+    // (slot -1: the wave's only piece, straight into the output); the split
+    // waves' pieces write FP64 partial slots [slot][64][3] that a combine
+    // launch adds up per split wave {wave, first slot, pieces}
+    int4* pieces = nullptr;
+    int64_t num_pieces = 0;
+    int4* split = nullptr;
+    int64_t num_split = 0;
+    double* planes = nullptr;
     int num_cus = 256;               // compute units of the device
     int64_t grid_blocks = 0;         // persistent grid size (0 = from the device)
 
@@ -231,6 +242,11 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
 // upward pass with them)
 int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st);
 void adjoint_plan_free(cagr_tree* t);
+// cuts the self-query walks into pieces of about piece_cost times the mean
+// wave cost (at most max_pieces a wave), costed by the walk on the FP64
+// centres of set 0 of the last upward pass; piece_cost <= 0 removes them
+int split_walks(cagr_tree* t, double piece_cost, int max_pieces, hipStream_t st);
+void split_walks_free(cagr_tree* t);
 int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuous, double an,
              void* out, int odtype, int64_t* cnt_far, int64_t* cnt_near, int64_t* cnt_visit,
              hipStream_t st);

@@ -38,7 +38,9 @@
 // evaluates exp per interaction like the reference.
 #include "common.h"
 #include "trav_common.h"
+#include <algorithm>
 #include <climits>
+#include <cmath>
 
 namespace cagr {
 namespace {
@@ -74,13 +76,32 @@ struct TP {
     const int32_t* tile_order;// regular grid: dispatch slot -> wave (null: identity)
     const double* r64;        // FP64 centres of the upward pass, [set][node][3] (retests)
     int64_t nm;               // nodes (set stride of r64)
+    // walk pieces (self queries of one tree; null: one wave per 64 queries
+    // on the whole tree): {wave, first node, end node, partial slot or -1}
+    const int4* pieces;
+    double* planes;           // partial slots: [slot][64 lanes][3] FP64
 };
+
+// Where a walk's results go: the output array (slot < 0), or (a piece of a
+// split walk) its partial slot in p.planes, 3 FP64 values per lane, which
+// k_piece_combine adds up
+struct Sink {
+    int32_t slot;
+    int lane;
+};
+__device__ inline void emit(const TP& p, const Sink& sk, int64_t oi, int no, int k, double v);
 
 using namespace dev;
 
 __device__ inline void store_out(const TP& p, int64_t i, double v) {
     if (p.odtype == CAGR_F32) ((float*)p.out)[i] = (float)v;
     else ((double*)p.out)[i] = v;
+}
+
+// value k of the no values of query oi
+__device__ inline void emit(const TP& p, const Sink& sk, int64_t oi, int no, int k, double v) {
+    if (sk.slot >= 0) p.planes[((int64_t)sk.slot * 64 + sk.lane) * 3 + k] = v;
+    else store_out(p, no * oi + k, v);
 }
 
 template <int QM>
@@ -207,16 +228,16 @@ struct VAcc64 {
 
 // outputs of one query of the vector family from its FP32c sums
 template <int VOP, bool CONT>
-__device__ inline void finish_v32(const TP& p, const VAcc32<VOP, CONT>& acc, int64_t oi,
-                                  const double* x, double w) {
+__device__ inline void finish_v32(const TP& p, const Sink& sk, const VAcc32<VOP, CONT>& acc,
+                                  int64_t oi, const double* x, double w) {
     double res[3];
     for (int k = 0; k < acc.NO; ++k) res[k] = acc.s[k];
     if constexpr (VOP == VOP_A) {
         double v = res[0];
         if constexpr (CONT) v += (double)acc.P / (w * w * w);
-        store_out(p, oi, p.sign * v);
+        emit(p, sk, oi, 1, 0, p.sign * v);
     } else if constexpr (VOP == VOP_G) {
-        for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, res[k]);
+        for (int k = 0; k < 3; ++k) emit(p, sk, oi, 3, k, res[k]);
     } else {
         double o[3];
         const double ew = CONT ? exp(p.an * w) : 0.0;
@@ -226,27 +247,34 @@ __device__ inline void finish_v32(const TP& p, const VAcc32<VOP, CONT>& acc, int
                 o[k] += ew * ((double)acc.P / (w * w * w) + 0.5 * p.an * (double)acc.Q[k] / w);
         }
         if constexpr (VOP == VOP_A3) {
-            store_out(p, oi, ((o[0] + o[1]) + o[2]) / 3.0);
+            emit(p, sk, oi, 1, 0, ((o[0] + o[1]) + o[2]) / 3.0);
         } else if (p.kdir >= 0) {
-            store_out(p, oi, sel3(o, p.kdir));
+            emit(p, sk, oi, 1, 0, sel3(o, p.kdir));
         } else {
-            for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, o[k]);
+            for (int k = 0; k < 3; ++k) emit(p, sk, oi, 3, k, o[k]);
         }
     }
 }
 
-// one wave: queries idx0 + lane (< qend), tree nodes [root, stop)
+// one wave: queries idx0 + lane (< qend), tree nodes [root, stop).  A piece
+// (chain = true) starts at a node root inside the tree: each lane first
+// replays the opening tests of root's ancestors, so that it is active at
+// root only if its own walk reaches root, and otherwise resumes after the
+// shallowest ancestor it takes far -- the piece then takes exactly the steps
+// of the whole walk that fall in [root, stop).  Results go to sk.
 template <int P, int VOP, int QM, bool CONT, bool CNT>
 __device__ __forceinline__ void trav_v(
     const TP& p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
-    const float* __restrict__ pts32, int64_t idx0, int64_t qend, int32_t root, int32_t stop) {
+    const float* __restrict__ pts32, int64_t idx0, int64_t qend, int32_t root, int32_t stop,
+    int32_t part = -1, bool chain = false) {
     const int lane = threadIdx.x & 63;
     const int64_t idx = idx0 + lane;
     double x[3], w;
     int64_t oi;
     const bool valid = load_query<QM>(p, pts64, idx, qend, x, w, oi);
     if (__ballot(valid) == 0) return;
+    const Sink sk{part, lane};
 
     int32_t resume = valid ? root : INT_MAX;
     int64_t nfar = 0, nnear = 0, nvisit = 0;
@@ -270,6 +298,24 @@ __device__ __forceinline__ void trav_v(
         const float w2 = wf * wf;
         const int dsafe = __builtin_amdgcn_readfirstlane(
             far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
+        if (chain) {   // the ancestors' opening tests, exactly as the steps below take them
+            for (int32_t a = rec32(topo, root).parent; a >= 0;) {
+                a = __builtin_amdgcn_readfirstlane(a);
+                const TopoRec ta = rec32(topo, a);
+                const NodeV32 na = rec32(nodes, a);
+                float d[3] = {xf[0] - na.cf[0], xf[1] - na.cf[1], xf[2] - na.cf[2]};
+                if constexpr (QM != QM_SELF)
+                    for (int k = 0; k < 3; ++k) d[k] += xl[k];
+                const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                bool far = r2 > na.hi2;
+                if (!far && !(r2 < na.lo2)) {
+                    const double* c64 = p.r64 + 3 * (int64_t)a;
+                    far = dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > ta.thr2;
+                }
+                if (far && valid) resume = ta.escape;
+                a = ta.parent;
+            }
+        }
         VAcc32<VOP, CONT> acc;
         acc.init();
         for (int32_t n = root; n < stop;) {
@@ -397,11 +443,22 @@ __device__ __forceinline__ void trav_v(
         }
         if constexpr (!CNT) {
             acc.flush_leaf();
-            if (valid) finish_v32<VOP, CONT>(p, acc, oi, x, w);
+            if (valid) finish_v32<VOP, CONT>(p, sk, acc, oi, x, w);
         }
     } else {
         const NodeV64* __restrict__ nodes = (const NodeV64*)nodes_;
         const SrcV64* __restrict__ src = (const SrcV64*)src_;
+        if (chain) {   // the ancestors' opening tests (see the FP32c walk)
+            for (int32_t a = rec32(topo, root).parent; a >= 0;) {
+                a = __builtin_amdgcn_readfirstlane(a);
+                const TopoRec ta = rec32(topo, a);
+                const NodeV64 na = rec32(nodes, a);
+                const bool far =
+                    dist2_rn(x[0] - na.c[0], x[1] - na.c[1], x[2] - na.c[2]) > ta.thr2;
+                if (far && valid) resume = ta.escape;
+                a = ta.parent;
+            }
+        }
         VAcc64<VOP, CONT> acc;
         acc.init();
         for (int32_t n = root; n < stop;) {
@@ -479,9 +536,9 @@ __device__ __forceinline__ void trav_v(
                 if constexpr (VOP == VOP_A) {
                     double v = acc.s[0];
                     if constexpr (CONT) v += acc.P / (w * w * w);
-                    store_out(p, oi, p.sign * v);
+                    emit(p, sk, oi, 1, 0, p.sign * v);
                 } else if constexpr (VOP == VOP_G) {
-                    for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, acc.s[k]);
+                    for (int k = 0; k < 3; ++k) emit(p, sk, oi, 3, k, acc.s[k]);
                 } else {
                     double o[3];
                     for (int k = 0; k < 3; ++k) {
@@ -490,11 +547,11 @@ __device__ __forceinline__ void trav_v(
                             o[k] += exp(p.an * w) * (acc.P / (w * w * w) + 0.5 * p.an * acc.Q[k] / w);
                     }
                     if constexpr (VOP == VOP_A3) {
-                        store_out(p, oi, ((o[0] + o[1]) + o[2]) / 3.0);
+                        emit(p, sk, oi, 1, 0, ((o[0] + o[1]) + o[2]) / 3.0);
                     } else if (p.kdir >= 0) {
-                        store_out(p, oi, sel3(o, p.kdir));
+                        emit(p, sk, oi, 1, 0, sel3(o, p.kdir));
                     } else {
-                        for (int k = 0; k < 3; ++k) store_out(p, 3 * oi + k, o[k]);
+                        for (int k = 0; k < 3; ++k) emit(p, sk, oi, 3, k, o[k]);
                     }
                 }
             }
@@ -512,17 +569,20 @@ __device__ __forceinline__ void trav_v(
 // ============================================================================
 // scalar family: AT (an = 0), A2Tk (K = 1, direction kdir), sum_k A2Tk (K = 3)
 // ============================================================================
+// (pieces: as trav_v, one resume per set)
 template <int P, int K, int QM, bool CNT>
 __device__ __forceinline__ void trav_s(
     const TP& p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
-    const float* __restrict__ pts32, int64_t idx0, int64_t qend, int32_t root, int32_t stop) {
+    const float* __restrict__ pts32, int64_t idx0, int64_t qend, int32_t root, int32_t stop,
+    int32_t part = -1, bool chain = false) {
     const int lane = threadIdx.x & 63;
     const int64_t idx = idx0 + lane;
     double x[3], w;
     int64_t oi;
     const bool valid = load_query<QM>(p, pts64, idx, qend, x, w, oi);
     if (__ballot(valid) == 0) return;
+    const Sink sk{part, lane};
 
     int32_t resume[K];
     for (int k = 0; k < K; ++k) resume[k] = valid ? root : INT_MAX;
@@ -549,6 +609,27 @@ __device__ __forceinline__ void trav_s(
         const float w2 = wf * wf;
         const int dsafe = __builtin_amdgcn_readfirstlane(
             far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
+        if (chain) {   // the ancestors' opening tests of every set (see trav_v)
+            for (int32_t a = rec32(topo, root).parent; a >= 0;) {
+                a = __builtin_amdgcn_readfirstlane(a);
+                const TopoRec ta = rec32(topo, a);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const NodeS32 na = rec32(nodes, a * K + k);
+                    float d[3] = {xf[0] - na.cf[0], xf[1] - na.cf[1], xf[2] - na.cf[2]};
+                    if constexpr (QM != QM_SELF)
+                        for (int cc = 0; cc < 3; ++cc) d[cc] += xl[cc];
+                    const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+                    bool far = r2 > na.hi2;
+                    if (!far && !(r2 < na.lo2)) {
+                        const double* c64 = p.r64 + 3 * ((int64_t)k * p.nm + a);
+                        far = dist2_rn(x[0] - c64[0], x[1] - c64[1], x[2] - c64[2]) > ta.thr2;
+                    }
+                    if (far && valid) resume[k] = ta.escape;
+                }
+                a = ta.parent;
+            }
+        }
         double s[3] = {0.0, 0.0, 0.0};   // FP64 totals of the FP32 partials
         float lp[3] = {0.f, 0.f, 0.f}, fp[3] = {0.f, 0.f, 0.f};
         // one near-pair interaction of all K sets (weights mk), into dst
@@ -778,13 +859,25 @@ __device__ __forceinline__ void trav_s(
         }
         if constexpr (!CNT) {
             if (valid)
-                for (int cc = 0; cc < 3; ++cc) {
-                    store_out(p, 3 * oi + cc, s[cc] + (double)fp[cc]);
-                }
+                for (int cc = 0; cc < 3; ++cc) emit(p, sk, oi, 3, cc, s[cc] + (double)fp[cc]);
         }
     } else {
         const NodeS64* __restrict__ nodes = (const NodeS64*)nodes_;
         const SrcS64<K>* __restrict__ src = (const SrcS64<K>*)src_;
+        if (chain) {   // the ancestors' opening tests of every set (see trav_v)
+            for (int32_t a = rec32(topo, root).parent; a >= 0;) {
+                a = __builtin_amdgcn_readfirstlane(a);
+                const TopoRec ta = rec32(topo, a);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const NodeS64 na = rec32(nodes, a * K + k);
+                    const bool far =
+                        dist2_rn(x[0] - na.c[0], x[1] - na.c[1], x[2] - na.c[2]) > ta.thr2;
+                    if (far && valid) resume[k] = ta.escape;
+                }
+                a = ta.parent;
+            }
+        }
         double acc[3] = {0.0, 0.0, 0.0};
         // one A2T_dir interaction (AT when an = 0) of scalar sv at d, width wq,
         // into dst
@@ -899,7 +992,7 @@ __device__ __forceinline__ void trav_s(
         }
         if constexpr (!CNT) {
             if (valid)
-                for (int cc = 0; cc < 3; ++cc) store_out(p, 3 * oi + cc, acc[cc]);
+                for (int cc = 0; cc < 3; ++cc) emit(p, sk, oi, 3, cc, acc[cc]);
         }
     }
     if constexpr (CNT) {
@@ -912,16 +1005,48 @@ __device__ __forceinline__ void trav_s(
 }
 
 // ---------------------------------------------------------------- kernels
-// one wave per 64 consecutive queries, every wave on the whole tree
+// what the wave in dispatch slot `slot` walks: a piece, or (no pieces) the
+// slot's wave in dispatch order on the whole tree
+struct WaveWork {
+    int64_t q0;
+    int32_t lo, hi;
+    int32_t part;
+    bool chain;
+};
+__device__ inline WaveWork wave_work(const TP& p, int64_t slot_) {
+    // the slot is the same for the whole wave: made a scalar, the piece or
+    // dispatch-order entry is a scalar load and its fields stay in scalar
+    // registers (a vector copy would turn the walk's loop bounds into
+    // per-lane values)
+    const int32_t slot = __builtin_amdgcn_readfirstlane((int32_t)slot_);
+    WaveWork ww{0, p.root, p.stop, -1, false};
+    if (p.pieces) {
+        const int4 pc = p.pieces[slot];
+        ww.q0 = (int64_t)pc.x * 64;
+        ww.lo = pc.y;
+        ww.hi = pc.z;
+        ww.part = pc.w;
+        ww.chain = pc.y != p.root;
+    } else {
+        ww.q0 = (p.tile_order ? (int64_t)p.tile_order[slot] : slot) * 64;
+    }
+    return ww;
+}
+
+// one wave per 64 consecutive queries, every wave on the whole tree; or, for
+// the self queries of a tree with walk pieces, one wave per piece
+// {wave, first node, end node, partial slot}: the walk of that wave over the
+// node range, into the output (slot -1) or its partial slot
 template <int P, int VOP, int QM, bool CONT, bool CNT>
 __global__ __launch_bounds__(kThreads) void k_trav_v(
     TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
     const float* __restrict__ pts32) {
-    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (p.tile_order) wave = p.tile_order[wave];
-    trav_v<P, VOP, QM, CONT, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root,
-                                  p.stop);
+    const int64_t slot =
+        xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const WaveWork ww = wave_work(p, slot);
+    trav_v<P, VOP, QM, CONT, CNT>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq, ww.lo, ww.hi,
+                                  ww.part, ww.chain);
 }
 
 template <int P, int K, int QM, bool CNT>
@@ -929,9 +1054,11 @@ __global__ __launch_bounds__(kThreads) void k_trav_s(
     TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
     const float* __restrict__ pts32) {
-    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (p.tile_order) wave = p.tile_order[wave];
-    trav_s<P, K, QM, CNT>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq, p.root, p.stop);
+    const int64_t slot =
+        xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const WaveWork ww = wave_work(p, slot);
+    trav_s<P, K, QM, CNT>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq, ww.lo, ww.hi, ww.part,
+                          ww.chain);
 }
 
 // The FP32c walks over the tree's own points, held to 64 VGPRs so that 8
@@ -943,10 +1070,11 @@ __global__ __launch_bounds__(kThreads, 8) void k_trav_v32(
     TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
     const float* __restrict__ pts32) {
-    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (p.tile_order) wave = p.tile_order[wave];
-    trav_v<CAGR_PREC_FP32C, VOP, QM, CONT, false>(p, topo, nodes_, src_, pts64, pts32, wave * 64,
-                                                  p.nq, p.root, p.stop);
+    const int64_t slot =
+        xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const WaveWork ww = wave_work(p, slot);
+    trav_v<CAGR_PREC_FP32C, VOP, QM, CONT, false>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq,
+                                                  ww.lo, ww.hi, ww.part, ww.chain);
 }
 
 template <int K, int QM>
@@ -954,10 +1082,11 @@ __global__ __launch_bounds__(kThreads, 8) void k_trav_s32(
     TP p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
     const float* __restrict__ pts32) {
-    int64_t wave = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (p.tile_order) wave = p.tile_order[wave];
-    trav_s<CAGR_PREC_FP32C, K, QM, false>(p, topo, nodes_, src_, pts64, pts32, wave * 64, p.nq,
-                                          p.root, p.stop);
+    const int64_t slot =
+        xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const WaveWork ww = wave_work(p, slot);
+    trav_s<CAGR_PREC_FP32C, K, QM, false>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq, ww.lo,
+                                          ww.hi, ww.part, ww.chain);
 }
 
 // Persistent form: a grid sized to the device pulls wave items {root, stop,
@@ -1041,6 +1170,77 @@ __global__ __launch_bounds__(512, 8) void k_trav_s_items32(
 
 __global__ void k_reset_counter(int32_t* c) {
     if (threadIdx.x < kQueues) c[threadIdx.x * kCounterStride] = 0;
+}
+
+// ---------------------------------------------------------------- walk pieces
+// The output of every split wave {wave, first slot, pieces}: the sum of its
+// pieces' partial slots in piece order (a fixed order, so the result does not
+// depend on when the pieces ran), no values per query.  One wave per split
+// wave.
+__global__ __launch_bounds__(64) void k_piece_combine(TP p, const int4* __restrict__ split,
+                                                      int no) {
+    const int4 sw = split[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int64_t s = (int64_t)sw.x * 64 + lane;
+    if (s >= p.nq) return;
+    const int64_t oi = p.perm[s];
+    for (int k = 0; k < no; ++k) {
+        double v = 0.0;
+        for (int j = 0; j < sw.z; ++j) v += p.planes[((int64_t)(sw.y + j) * 64 + lane) * 3 + k];
+        store_out(p, no * oi + k, v);
+    }
+}
+
+// Cost of the steps of each wave's self-query walk (the decisions of the FP64
+// test on the FP64 centres c64 [m, 3]): 1 per node step, plus kNearLeafCost
+// per leaf step with near lanes (a near leaf's points, measured at about two
+// to three node steps a leaf).  np == null: the total of each wave into
+// total.  Else, for a wave cut into np[w] pieces, the node at which the cost
+// before the step first reaches c * total[w] / np[w] (c = 1 .. np[w]-1) into
+// cuts[w * pmax + c - 1] (m where it never does: an empty last piece).
+constexpr int kNearLeafCost = 3;
+__global__ __launch_bounds__(64) void k_walk_cost(const TopoRec* __restrict__ topo,
+                                                  const double* __restrict__ c64,
+                                                  const double* __restrict__ pts64, int64_t n,
+                                                  int32_t m, const int32_t* __restrict__ np,
+                                                  int32_t* __restrict__ total,
+                                                  int32_t* __restrict__ cuts, int pmax) {
+    const int64_t wave = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t s = wave * 64 + lane;
+    const bool valid = s < n;
+    const int64_t sc = valid ? s : 0;
+    const double x[3] = {pts64[3 * sc], pts64[3 * sc + 1], pts64[3 * sc + 2]};
+    const int npw = np ? np[wave] : 1;
+    const int64_t tot = np ? total[wave] : 0;
+    if (np && lane < npw - 1) cuts[wave * pmax + lane] = m;
+    int32_t resume = valid ? 0 : INT_MAX;
+    int64_t cost = 0;
+    int c = 1;
+    for (int32_t nd = 0; nd < m;) {
+        nd = __builtin_amdgcn_readfirstlane(nd);
+        while (c < npw && cost * npw >= c * tot) {
+            if (lane == 0) cuts[wave * pmax + c - 1] = nd;
+            ++c;
+        }
+        const TopoRec t = rec32(topo, nd);
+        const double* cn = c64 + 3 * (int64_t)nd;
+        const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= nd);
+        const uint64_t fm = __builtin_amdgcn_ballot_w64(
+            dist2_rn(x[0] - cn[0], x[1] - cn[1], x[2] - cn[2]) > t.thr2);
+        const uint64_t far_m = am & fm, near_m = am & ~fm;
+        cost += 1;
+        if (topo_leaf(t)) {
+            if (near_m != 0) cost += kNearLeafCost;
+            nd = t.escape;
+        } else if (near_m != 0) {
+            if (__builtin_amdgcn_inverse_ballot_w64(far_m)) resume = t.escape;
+            nd = nd + 1;
+        } else {
+            nd = t.escape;
+        }
+    }
+    if (!np && lane == 0) total[wave] = (int32_t)min(cost, (int64_t)INT_MAX);
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -1220,7 +1420,7 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     tp.nq = lanes;
     if (lanes == 0) return CAGR_OK;
     const int64_t waves = (lanes + 63) / 64;
-    const int64_t blocks = (waves + t->grid_waves - 1) / t->grid_waves;
+    int64_t blocks = (waves + t->grid_waves - 1) / t->grid_waves;
     const bool cnt = cnt_far || cnt_near || cnt_visit;
     const bool p64 = precision == CAGR_PREC_FP64;
     if (q.mode == QM_SELF && (t->persistent || t->num_clouds > 1)) {
@@ -1236,8 +1436,23 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     else if (p64) rc = launch_all<CAGR_PREC_FP64, QMV, false>(tp, t, op, continuous, blocks, st); \
     else rc = launch_all<CAGR_PREC_FP32C, QMV, false>(tp, t, op, continuous, blocks, st);
     if (q.mode == QM_SELF) {
-        if (t->grid_waves == 1 && !cnt) tp.tile_order = t->tile_order;
+        const bool pieces = t->grid_waves == 1 && !cnt && t->num_pieces > 0;
+        if (pieces) {
+            tp.pieces = t->pieces;
+            tp.planes = t->planes;
+            blocks = t->num_pieces;
+        } else if (t->grid_waves == 1 && !cnt) {
+            tp.tile_order = t->tile_order;
+        }
         CAGR_LAUNCH_QM(QM_SELF)
+        if (rc == CAGR_OK && pieces && t->num_split > 0) {
+            // values per query of the op
+            const int no = (op == CAGR_OP_G || op == CAGR_OP_A2 || op == CAGR_OP_AT ||
+                            op == CAGR_OP_A2T_SUM || (op >= CAGR_OP_A2T1 && op <= CAGR_OP_A2T3))
+                               ? 3 : 1;
+            k_piece_combine<<<(unsigned)t->num_split, 64, 0, st>>>(tp, t->split, no);
+            CAGR_LAUNCHED();
+        }
     } else if (q.mode == QM_EXPLICIT) {
         CAGR_LAUNCH_QM(QM_EXPLICIT)
     } else {
@@ -1245,6 +1460,106 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     }
 #undef CAGR_LAUNCH_QM
     return rc;
+}
+
+void split_walks_free(cagr_tree* t) {
+    (void)hipFree(t->pieces);
+    (void)hipFree(t->split);
+    (void)hipFree(t->planes);
+    t->pieces = nullptr;
+    t->split = nullptr;
+    t->planes = nullptr;
+    t->num_pieces = t->num_split = 0;
+}
+
+int split_walks(cagr_tree* t, double piece_cost, int max_pieces, hipStream_t st) {
+    split_walks_free(t);
+    if (!(piece_cost > 0.0) || t->n == 0 || t->num_clouds != 1) return CAGR_OK;
+    CAGR_REQUIRE(max_pieces >= 1 && max_pieces <= 64, CAGR_ERR_ARG, "max_pieces must be in [1, 64]");
+    const int64_t nw = (t->n + 63) / 64;
+    const int32_t m = (int32_t)t->m;
+    int32_t *dtot = nullptr, *dnp = nullptr, *dcuts = nullptr;
+    struct Free {
+        int32_t **a, **b, **c;
+        ~Free() { (void)hipFree(*a); (void)hipFree(*b); (void)hipFree(*c); }
+    } free_{&dtot, &dnp, &dcuts};
+    CAGR_HIP(hipMalloc(&dtot, nw * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&dnp, nw * sizeof(int32_t)));
+    CAGR_HIP(hipMalloc(&dcuts, nw * max_pieces * sizeof(int32_t)));
+    k_walk_cost<<<(unsigned)nw, 64, 0, st>>>(t->topo, t->r64, t->pts64, t->n, m, nullptr, dtot,
+                                              nullptr, max_pieces);
+    CAGR_LAUNCHED();
+    std::vector<int32_t> tot(nw), np(nw, 1), cuts(nw * max_pieces, m);
+    CAGR_HIP(hipMemcpyAsync(tot.data(), dtot, nw * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    double mean = 0.0;
+    for (int32_t c : tot) mean += c;
+    mean /= (double)nw;
+    const double target = std::max(1.0, piece_cost * mean);
+    int64_t nsplit = 0, slots = 0;
+    for (int64_t w = 0; w < nw; ++w) {
+        np[w] = (int32_t)std::min<double>(max_pieces, std::max(1.0, std::ceil(tot[w] / target)));
+        if (np[w] > 1) {
+            ++nsplit;
+            slots += np[w];
+        }
+    }
+    if (nsplit == 0) return CAGR_OK;
+    CAGR_HIP(hipMemcpyAsync(dnp, np.data(), nw * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    k_walk_cost<<<(unsigned)nw, 64, 0, st>>>(t->topo, t->r64, t->pts64, t->n, m, dnp, dtot, dcuts,
+                                              max_pieces);
+    CAGR_LAUNCHED();
+    CAGR_HIP(hipMemcpyAsync(cuts.data(), dcuts, nw * max_pieces * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    // pieces in wave (Morton) order, then, per XCD's contiguous range of
+    // dispatch slots (one wave per block, block b on XCD b mod 8), longest
+    // first
+    std::vector<int4> pcs;
+    std::vector<double> pcost;
+    std::vector<int4> split;
+    pcs.reserve(nw + slots);
+    int32_t slot = 0;
+    for (int64_t w = 0; w < nw; ++w) {
+        const int P = np[w];
+        if (P == 1) {
+            pcs.push_back(make_int4((int)w, 0, m, -1));
+            pcost.push_back(tot[w]);
+            continue;
+        }
+        split.push_back(make_int4((int)w, slot, P, 0));
+        for (int j = 0; j < P; ++j) {
+            const int32_t lo = j ? cuts[w * max_pieces + j - 1] : 0;
+            const int32_t hi = j + 1 < P ? cuts[w * max_pieces + j] : m;
+            pcs.push_back(make_int4((int)w, lo, std::max(lo, hi), slot + j));
+            pcost.push_back((double)tot[w] / P);
+        }
+        slot += P;
+    }
+    const int64_t G = (int64_t)pcs.size();
+    std::vector<int32_t> ord(G);
+    for (int64_t i = 0; i < G; ++i) ord[i] = (int32_t)i;
+    const int64_t q = G / kQueues, r = G % kQueues;
+    int64_t lo = 0;
+    for (int x = 0; x < kQueues; ++x) {
+        const int64_t len = q + (x < r ? 1 : 0);
+        std::stable_sort(ord.begin() + lo, ord.begin() + lo + len,
+                         [&](int32_t a, int32_t b) { return pcost[a] > pcost[b]; });
+        lo += len;
+    }
+    std::vector<int4> sorted(G);
+    for (int64_t i = 0; i < G; ++i) sorted[i] = pcs[ord[i]];
+    // the tree holds them (and split_walks_free releases them) from here on,
+    // counts set only once everything is in place
+    CAGR_HIP(hipMalloc(&t->pieces, G * sizeof(int4)));
+    CAGR_HIP(hipMalloc(&t->split, nsplit * sizeof(int4)));
+    CAGR_HIP(hipMalloc(&t->planes, (size_t)slots * 192 * sizeof(double)));
+    CAGR_HIP(hipMemcpyAsync(t->pieces, sorted.data(), G * sizeof(int4), hipMemcpyHostToDevice, st));
+    CAGR_HIP(hipMemcpyAsync(t->split, split.data(), nsplit * sizeof(int4), hipMemcpyHostToDevice, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    t->num_pieces = G;
+    t->num_split = nsplit;
+    return CAGR_OK;
 }
 
 }  // namespace cagr

@@ -74,6 +74,7 @@ class WindingNumberTreecode:
                                          _dt(self.points), max_tree_depth, self.leaf_size,
                                          self.theta, _stream(self.device), ct.byref(self._h)))
         self._ref_layout = None
+        self.walk_pieces = (0, 0)   # (pieces, waves cut) of the self-query walks (balance)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -191,22 +192,33 @@ class WindingNumberTreecode:
                                             _ptr(out), _stream(self.device)))
         return out
 
-    def balance(self, op: str = "A2", attrs: Optional[torch.Tensor] = None) -> np.ndarray:
+    def balance(self, op: str = "A2", attrs: Optional[torch.Tensor] = None, *,
+                piece_cost: float = 0.5, max_pieces: int = 8) -> np.ndarray:
         """Dispatch the traversal's waves longest first (within each XCD's part
         of the tree; the batched form's work items likewise), costed by the
         exact interaction counts of `op` on `attrs` (default: unit
-        attributes).  Returns the per-query cost used."""
+        attributes), and cut the walks that cost more than `piece_cost` times
+        the mean into node-range pieces (at most `max_pieces` a wave; 0: no
+        pieces), so that the longest walks do not set the launch's length.
+        Returns the per-query cost used."""
         if attrs is None:
             attrs = torch.ones((self.points.shape[0], IN_DIM[op]), dtype=self.points.dtype,
                                device=self.device)
         far, near, _ = self.counts(op, attrs)
         cost = (far + near).cpu().numpy().astype(np.int64)
         check(self.L.cagr_order_items(self._h, cost.ctypes.data_as(ct.c_void_p)))
+        # the pieces are costed on the centres of the counting pass's upward pass
+        cnt = np.zeros(2, np.int64)
+        with torch.cuda.device(self.device):
+            check(self.L.cagr_split_walks(self._h, float(piece_cost), int(max_pieces),
+                                          cnt.ctypes.data_as(ct.c_void_p), _stream(self.device)))
+        self.walk_pieces = (int(cnt[0]), int(cnt[1]))   # (pieces, waves cut)
         return cost
 
     def unbalance(self) -> None:
-        """Back to Morton dispatch order."""
+        """Back to Morton dispatch order, uncut walks."""
         check(self.L.cagr_order_items(self._h, None))
+        self.walk_pieces = (0, 0)
 
     def adjoint_plan(self, weights: torch.Tensor) -> None:
         """Record the far-field step sequence of the FP32 adjoint for the node

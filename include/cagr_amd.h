@@ -233,8 +233,24 @@ int cagr_forest_roots(const cagr_tree* tree, int64_t* node_offsets);
 int cagr_set_launch(cagr_tree* tree, int persistent, int waves_per_block, int64_t grid_blocks);
 /* Longest-first order of the persistent work items from a per-query cost
  * (HOST int64 [n], original order, e.g. n_far + n_near from cagr_count);
- * NULL restores the natural order. */
+ * NULL restores the natural order (and removes walk pieces). */
 int cagr_order_items(cagr_tree* tree, const int64_t* query_cost);
+/* Walk pieces (no counterpart in the reference; a load-balancing setting of
+ * this library): the self-query walk of a wave (64 consecutive sorted points)
+ * whose cost exceeds piece_cost times the mean wave cost is cut into at most
+ * max_pieces (in [1, 64]) contiguous preorder node ranges of about equal
+ * cost, each run as its own wave, so that the longest walks no longer set a
+ * launch's length; the pieces of a wave add up in a fixed order, so results
+ * do not depend on their timing (they differ from the uncut walk's only by
+ * the grouping of the sums).  Costed by the walk on the FP64 node centres of
+ * the most recent upward pass (set 0: e.g. the one of cagr_count), pieces
+ * dispatched longest first within each XCD's share.  Used by the
+ * regular-grid self-query traversals of single-cloud trees (one wave per
+ * block); piece_cost <= 0 removes them.  counts (HOST int64 [2], may be
+ * NULL) receives the number of pieces and of waves cut.  Synchronises the
+ * stream. */
+int cagr_split_walks(cagr_tree* tree, double piece_cost, int max_pieces, int64_t* counts,
+                     void* stream);
 
 /* Opt-in timing of the library's own launches on the caller's stream (no
  * counterpart in the reference, which records no timings).  enable = 1 clears

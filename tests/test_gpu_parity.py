@@ -137,6 +137,62 @@ def test_every_operator_matches_oracle(cloud, leaf):
         assert np.array_equal(gv.cpu().numpy(), cv)
 
 
+@pytest.mark.parametrize("leaf", [1, 32])
+def test_walk_pieces_match_oracle_and_whole_walks(leaf):
+    """Walks cut into node-range pieces (balance with a small piece cost, so
+    that most waves are cut, into up to 16 pieces each) give the oracle's sums
+    for every operator in both precisions, and the uncut walks' to the
+    rounding of the regrouped sums; unbalancing restores the uncut results
+    bit for bit."""
+    p = synth.normalize(synth.ellipsoids(40000, 5)).astype(np.float32).astype(np.float64)
+    n = p.shape[0]
+    rng = np.random.default_rng(30 + leaf)
+    wt = WindingNumberTreecode(t(p), leaf_size=leaf)
+    ot = oracle.build_tree(p, 15, leaf)
+    widths = rng.uniform(0.002, 0.01, n)
+    mu = synth.random_normals(n, 31)
+    s3 = rng.normal(size=(n, 3))
+    cases = ([(op, False) for op in OPS + ["A2", "A2T_sum"]]
+             + [("A3", True), ("A", True), ("A22", True)])
+
+    def attrs(op):
+        if op in ("A2T_sum", "A2"):
+            return s3 if op == "A2T_sum" else mu
+        return mu if oracle.IN_DIM[op] == 3 else s3[:, :1]
+
+    def run():
+        return {(op, c, prec): wt.apply(op, t(attrs(op)), t(widths), continuous=c,
+                                        precision=prec).cpu().numpy()
+                for op, c in cases for prec in ("fp64", "fp32c")}
+
+    whole = run()
+    wt.balance("A2", t(mu), piece_cost=0.15, max_pieces=16)
+    pieces, cut = wt.walk_pieces
+    waves = (n + 63) // 64
+    say("pieces", leaf, pieces, "pieces,", cut, "of", waves, "waves cut")
+    assert cut > waves // 2 and pieces >= waves + cut
+    cutr = run()
+    for op, c in cases:
+        if op in ("A2", "A2T_sum"):
+            if op == "A2":
+                ref = np.concatenate([oracle.apply(f"A2{k}", ot, p, mu, widths, continuous=c)
+                                      for k in (1, 2, 3)], 1)
+            else:
+                ref = sum(oracle.apply(f"A2T{k + 1}", ot, p, s3[:, k:k + 1], widths)
+                          for k in range(3))
+        else:
+            ref = oracle.apply(op, ot, p, attrs(op), widths, continuous=c)
+        for prec, tol, same in (("fp64", TOL64, 1e-13), ("fp32c", TOL32C, 1e-6)):
+            out = cutr[(op, c, prec)]
+            assert normwise(out, ref) <= tol, (op, c, prec)
+            assert normwise(out, whole[(op, c, prec)]) <= same, (op, c, prec)
+    wt.unbalance()
+    assert wt.walk_pieces == (0, 0)
+    again = run()
+    for k in whole:
+        assert np.array_equal(again[k], whole[k]), k
+
+
 @pytest.mark.parametrize("theta", [0.4, 0.6])
 def test_theta_variants(theta):
     p = synth.normalize(synth.ellipsoids(4000, 9)).astype(np.float32).astype(np.float64)
