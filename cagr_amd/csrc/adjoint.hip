@@ -29,6 +29,10 @@ constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / 64;
 
 enum { GM_A = 0, GM_A2K = 1, GM_A3 = 2, GM_A2 = 3 };
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt at their maxima, gfx9 encoding):
+// a list chunk's loads are complete before its steps, so the steps' loop
+// carries no wait for them (nor for the steps' own stores)
+constexpr int kWaitVm0 = 0x0F70;
 // modes of the FP32 walk: the plain adjoint (far sums as FP64 atomics into L),
 // and the two walks that build the adjoint plan (count, then record)
 enum { FM_ATOMIC = 0, FM_COUNT = 2, FM_FILL = 3 };
@@ -71,8 +75,11 @@ struct AP {
 // the wave (64 consecutive sorted points) this block serves: blocks of one
 // wave, dispatched XCD-aware, in the tree's longest-first order when it has one
 __device__ inline int64_t wave_of(const AP& p) {
-    const int64_t slot = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    return p.tile_order ? (int64_t)p.tile_order[slot] : slot;
+    // a scalar: the order entry is a scalar load and the wave index stays in
+    // a scalar register
+    const int32_t slot = __builtin_amdgcn_readfirstlane(
+        (int32_t)(xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    return p.tile_order ? (int64_t)p.tile_order[slot] : (int64_t)slot;
 }
 
 // cotangent of query oi as the per-direction weights of the k-sum form
@@ -398,52 +405,58 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
         far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
 
     // the lists are read 64 entries at a time, one per lane (coalesced), and
-    // entry k is taken from lane k: no dependent load in front of each step
+    // entry k of a chunk is taken from lane k: no dependent load in front of
+    // each step.  The chunk's loads are waited for once, in front of its
+    // steps; a step's slot store is not waited for (a load issued inside the
+    // steps' loop would make every step wait for all outstanding memory
+    // operations, the previous step's store among them)
     const int32_t slot0 = __builtin_amdgcn_readfirstlane(p.wave_off[wave]);
     const int32_t nf = __builtin_amdgcn_readfirstlane(p.wave_off[wave + 1]) - slot0;
-    int32_t node_l = 0, pos_l = 0;
-    uint64_t mask_l = 0;
-    for (int32_t k = 0; k < nf; ++k) {
-        if ((k & 63) == 0) {
-            const int32_t kl = k + lane;
-            node_l = kl < nf ? p.far_node[slot0 + kl] : 0;
-            mask_l = kl < nf ? p.far_mask[slot0 + kl] : 0;
-            pos_l = kl < nf ? p.slot_pos[slot0 + kl] : 0;
+    for (int32_t k0 = 0; k0 < nf; k0 += 64) {
+        const int32_t kl = k0 + lane;
+        const int32_t node_l = kl < nf ? p.far_node[slot0 + kl] : 0;
+        const uint64_t mask_l = kl < nf ? p.far_mask[slot0 + kl] : 0;
+        const int32_t pos_l = kl < nf ? p.slot_pos[slot0 + kl] : 0;
+        const int32_t kn = min(64, nf - k0);
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        for (int32_t k = 0; k < kn; ++k) {
+            const int32_t n = __builtin_amdgcn_readlane(node_l, k);
+            const uint64_t m = lane_bcast_u64(mask_l, k);
+            const int32_t pos = __builtin_amdgcn_readlane(pos_l, k);
+            const bool take = __builtin_amdgcn_inverse_ballot_w64(m);
+            const TopoRec t = rec32(topo, n);
+            const NodeV32 nr = rec32(nodes, n);
+            CAGR_LOADS_HERE("s"(t.info), "s"(nr.cf[0]), "s"(nr.cf[1]), "s"(nr.cf[2]), "s"(nr.e[0]),
+                            "s"(nr.e[1]), "s"(nr.e[2]));
+            __builtin_amdgcn_sched_barrier(0);
+            const float d[3] = {xf[0] - nr.cf[0], xf[1] - nr.cf[1], xf[2] - nr.cf[2]};
+            const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+            bool in = false;   // far at depth <= dsafe proves r > w
+            if (topo_depth(t) > dsafe)
+                in = inside_centre(r2, w2, wf, nr.ce, x, p.c64 + 3 * (int64_t)n, w, take);
+            float z[3] = {0.f, 0.f, 0.f};
+            adj_term32<GA, CONT>(z, d, r2, take, in, h, nr.e, hanf, gin, winv, ew);
+            // the three totals stay in their rows (row 0: z0, 1: z2, 2: z1);
+            // the first lane of each row stores its component of the slot
+            const float tot = wave_sum3_rows(z[0], z[1], z[2]);
+            if ((lane & 15) == 0 && lane < 48)
+                ((float*)p.far_val)[4 * (int64_t)pos + (lane == 0 ? 0 : (lane == 16 ? 2 : 1))] = tot;
         }
-        const int32_t n = __builtin_amdgcn_readlane(node_l, k & 63);
-        const uint64_t m = lane_bcast_u64(mask_l, k & 63);
-        const int32_t pos = __builtin_amdgcn_readlane(pos_l, k & 63);
-        const bool take = __builtin_amdgcn_inverse_ballot_w64(m);
-        const TopoRec t = rec32(topo, n);
-        const NodeV32 nr = rec32(nodes, n);
-        CAGR_LOADS_HERE("s"(t.info), "s"(nr.cf[0]), "s"(nr.cf[1]), "s"(nr.cf[2]), "s"(nr.e[0]),
-                        "s"(nr.e[1]), "s"(nr.e[2]));
-        __builtin_amdgcn_sched_barrier(0);
-        const float d[3] = {xf[0] - nr.cf[0], xf[1] - nr.cf[1], xf[2] - nr.cf[2]};
-        const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-        bool in = false;   // far at depth <= dsafe proves r > w
-        if (topo_depth(t) > dsafe)
-            in = inside_centre(r2, w2, wf, nr.ce, x, p.c64 + 3 * (int64_t)n, w, take);
-        float z[3] = {0.f, 0.f, 0.f};
-        adj_term32<GA, CONT>(z, d, r2, take, in, h, nr.e, hanf, gin, winv, ew);
-        float s0, s1, s2;
-        wave_sum3(z[0], z[1], z[2], s0, s1, s2);
-        if (lane == 0) p.far_val[pos] = make_float4(s0, s1, s2, 0.f);
     }
     const int32_t near0 = __builtin_amdgcn_readfirstlane(p.near_off[wave]);
     const int32_t nn = __builtin_amdgcn_readfirstlane(p.near_off[wave + 1]) - near0;
-    int32_t leaf_l = 0;
-    uint64_t nmask_l = 0;
-    for (int32_t j = 0; j < nn; ++j) {
-        if ((j & 63) == 0) {
-            const int32_t jl = j + lane;
-            leaf_l = jl < nn ? p.near_leaf[near0 + jl] : 0;
-            nmask_l = jl < nn ? p.near_mask[near0 + jl] : 0;
+    for (int32_t j0 = 0; j0 < nn; j0 += 64) {
+        const int32_t jl = j0 + lane;
+        const int32_t leaf_l = jl < nn ? p.near_leaf[near0 + jl] : 0;
+        const uint64_t nmask_l = jl < nn ? p.near_mask[near0 + jl] : 0;
+        const int32_t jn = min(64, nn - j0);
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        for (int32_t j = 0; j < jn; ++j) {
+            const int32_t n = __builtin_amdgcn_readlane(leaf_l, j);
+            const uint64_t qmask = lane_bcast_u64(nmask_l, j);
+            const TopoRec t = rec32(topo, n);
+            adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
         }
-        const int32_t n = __builtin_amdgcn_readlane(leaf_l, j & 63);
-        const uint64_t qmask = lane_bcast_u64(nmask_l, j & 63);
-        const TopoRec t = rec32(topo, n);
-        adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
     }
 }
 

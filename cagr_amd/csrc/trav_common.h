@@ -68,12 +68,29 @@ __device__ inline float dpp_move(float v) {
                                                                  ROWMASK, 0xf, false));
 }
 
-// the sums over the wave's 64 lanes of three values, as scalars: the lane
-// halves are exchanged with v_permlane32_swap (z0 with z1, z2 with itself)
-// and added, the rows of 16 lanes with v_permlane16_swap, then each row is
-// reduced by DPP moves -- row 0 then holds z0's total, row 1 z2's, row 2
-// z1's.  13 instructions against 21 for three separate DPP reductions.  All
-// 64 lanes must be active.
+// the sums over the wave's 64 lanes of three values, left in the rows of 16
+// lanes: the lane halves are exchanged with v_permlane32_swap (z0 with z1, z2
+// with itself) and added, the rows of 16 lanes with v_permlane16_swap, then
+// each row is reduced by DPP moves -- every lane of row 0 then holds z0's
+// total, of row 1 z2's, of row 2 z1's.  All 64 lanes must be active.
+__device__ inline float wave_sum3_rows(float z0, float z1, float z2) {
+    auto bits = [](float v) { return __builtin_bit_cast(unsigned, v); };
+    auto flt = [](unsigned v) { return __builtin_bit_cast(float, v); };
+    const auto a = __builtin_amdgcn_permlane32_swap(bits(z0), bits(z1), false, false);
+    const float u = flt(a[0]) + flt(a[1]);   // lanes 0-31: z0[l] + z0[l+32]; 32-63: z1's
+    const auto b = __builtin_amdgcn_permlane32_swap(bits(z2), bits(z2), false, false);
+    const float v = flt(b[0]) + flt(b[1]);   // every lane l: z2[l % 32] + z2[l % 32 + 32]
+    const auto c = __builtin_amdgcn_permlane16_swap(bits(u), bits(v), false, false);
+    float w = flt(c[0]) + flt(c[1]);         // rows: u 0+1, v 0+1, u 2+3, v 2+3
+    w += dpp_move<0xb1, 0xf>(w);    // quad_perm [1,0,3,2]
+    w += dpp_move<0x4e, 0xf>(w);    // quad_perm [2,3,0,1]
+    w += dpp_move<0x141, 0xf>(w);   // row_half_mirror
+    w += dpp_move<0x140, 0xf>(w);   // row_mirror: every lane holds its row's sum
+    return w;
+}
+
+// the same totals as scalars (13 instructions against 21 for three separate
+// DPP reductions)
 __device__ inline void wave_sum3(float z0, float z1, float z2, float& s0, float& s1, float& s2) {
     auto bits = [](float v) { return __builtin_bit_cast(unsigned, v); };
     auto flt = [](unsigned v) { return __builtin_bit_cast(float, v); };
