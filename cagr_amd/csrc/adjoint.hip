@@ -166,7 +166,7 @@ __device__ inline int near_query(const NearGroups& g, int b0, bool& ok) {
 }
 
 // The transposed near block of one leaf for the wave's near lanes qmask
-// (group layout above); the groups' sums meet in group 0 through a butterfly.
+// (group layout above); the groups' sums meet in every group (group_sum).
 // One FP64 atomic per source and component into N.  All 64 lanes must be
 // active.
 template <bool GA, bool CONT>
@@ -212,8 +212,7 @@ __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qma
             }
             adj_term32<GA, CONT>(zz, d, r2, use, in, qh, sr.e, hanf, qg, qwinv, qew);
         }
-        for (int sh = S; sh < 64; sh <<= 1)
-            for (int k = 0; k < 3; ++k) zz[k] += __shfl_xor(zz[k], sh, 64);
+        for (int k = 0; k < 3; ++k) zz[k] = group_sum(zz[k], S);
         if (has && ng.grp == 0) {
             atomicAdd(p.adj_pt + j, (double)zz[0]);
             atomicAdd(p.adj_pt + p.n + j, (double)zz[1]);

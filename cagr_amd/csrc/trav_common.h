@@ -89,6 +89,29 @@ __device__ inline float wave_sum3_rows(float z0, float z1, float z2) {
     return w;
 }
 
+// v summed over the lanes that agree in lane mod S (S a power of two <= 64),
+// the total in every lane: rotations by S..8 inside rows of 16 lanes (DPP
+// row_ror, one vector add each; a rotation by k keeps lane mod k), then the
+// row pairs and the wave halves exchanged with v_permlane16_swap and
+// v_permlane32_swap.  All 64 lanes must be active.
+__device__ inline float group_sum(float v, int S) {
+    auto bits = [](float u) { return __builtin_bit_cast(unsigned, u); };
+    auto flt = [](unsigned u) { return __builtin_bit_cast(float, u); };
+    if (S <= 1) v += dpp_move<0x121, 0xf>(v);   // row_ror:1
+    if (S <= 2) v += dpp_move<0x122, 0xf>(v);   // row_ror:2
+    if (S <= 4) v += dpp_move<0x124, 0xf>(v);   // row_ror:4
+    if (S <= 8) v += dpp_move<0x128, 0xf>(v);   // row_ror:8
+    if (S <= 16) {
+        const auto c = __builtin_amdgcn_permlane16_swap(bits(v), bits(v), false, false);
+        v = flt(c[0]) + flt(c[1]);
+    }
+    if (S <= 32) {
+        const auto c = __builtin_amdgcn_permlane32_swap(bits(v), bits(v), false, false);
+        v = flt(c[0]) + flt(c[1]);
+    }
+    return v;
+}
+
 // the same totals as scalars (13 instructions against 21 for three separate
 // DPP reductions)
 __device__ inline void wave_sum3(float z0, float z1, float z2, float& s0, float& s1, float& s2) {
