@@ -19,7 +19,9 @@
 // (A2Tk(An) = (-A2k(-An))^T).
 #include "common.h"
 #include "trav_common.h"
+#include <algorithm>
 #include <climits>
+#include <cmath>
 
 namespace cagr {
 namespace {
@@ -63,6 +65,7 @@ struct AP {
     uint64_t* slot_node;      // [slots] node of each slot (filling walk)
     int32_t* slot_id;         // [slots] the slot's own index (filling walk)
     float4* far_val;          // [slots] wave sums of the far steps
+    const int32_t* pieces;    // replay pieces, 8 ints each (null: whole waves)
     // the walk itself, recorded (filling walk) and replayed (list kernels)
     int32_t* far_node;        // [slots] node of each far step, wave-major
     uint64_t* far_mask;       // [slots] lanes that take it
@@ -364,7 +367,9 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
 
 // The planned adjoint: the recorded walk replayed.  Far steps come from the
 // wave's list (node, lanes) and need no opening test; their wave sums go to
-// the slots; near leaves come from the wave's near list.
+// the slots; near leaves come from the wave's near list.  With pieces, a
+// wave's lists are cut into ranges replayed by separate waves (the slots
+// and the near atomics do not depend on which wave takes a step).
 template <int GMODE_A, bool CONT>
 __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __restrict__ topo,
                                                          const NodeV32* __restrict__ nodes,
@@ -373,7 +378,24 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
                                                          const float* __restrict__ pts32) {
     constexpr bool GA = GMODE_A != 0;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = __builtin_amdgcn_readfirstlane((int)wave_of(p));
+    int64_t wave;
+    int32_t far_lo, far_hi, near_lo, near_hi;
+    if (p.pieces) {
+        const int32_t slot = __builtin_amdgcn_readfirstlane(
+            (int32_t)(xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+        const int32_t* pc = p.pieces + 8 * (int64_t)slot;
+        wave = pc[0];
+        far_lo = pc[1];
+        far_hi = pc[2];
+        near_lo = pc[3];
+        near_hi = pc[4];
+    } else {
+        wave = __builtin_amdgcn_readfirstlane((int)wave_of(p));
+        far_lo = __builtin_amdgcn_readfirstlane(p.wave_off[wave]);
+        far_hi = __builtin_amdgcn_readfirstlane(p.wave_off[wave + 1]);
+        near_lo = __builtin_amdgcn_readfirstlane(p.near_off[wave]);
+        near_hi = __builtin_amdgcn_readfirstlane(p.near_off[wave + 1]);
+    }
     const int64_t s = wave * 64 + lane;
     const bool valid = s < p.n;
     if (__ballot(valid) == 0) return;
@@ -409,8 +431,8 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
     // steps; a step's slot store is not waited for (a load issued inside the
     // steps' loop would make every step wait for all outstanding memory
     // operations, the previous step's store among them)
-    const int32_t slot0 = __builtin_amdgcn_readfirstlane(p.wave_off[wave]);
-    const int32_t nf = __builtin_amdgcn_readfirstlane(p.wave_off[wave + 1]) - slot0;
+    const int32_t slot0 = far_lo;
+    const int32_t nf = far_hi - far_lo;
     for (int32_t k0 = 0; k0 < nf; k0 += 64) {
         const int32_t kl = k0 + lane;
         const int32_t node_l = kl < nf ? p.far_node[slot0 + kl] : 0;
@@ -442,8 +464,8 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
                 ((float*)p.far_val)[4 * (int64_t)pos + (lane == 0 ? 0 : (lane == 16 ? 2 : 1))] = tot;
         }
     }
-    const int32_t near0 = __builtin_amdgcn_readfirstlane(p.near_off[wave]);
-    const int32_t nn = __builtin_amdgcn_readfirstlane(p.near_off[wave + 1]) - near0;
+    const int32_t near0 = near_lo;
+    const int32_t nn = near_hi - near_lo;
     for (int32_t j0 = 0; j0 < nn; j0 += 64) {
         const int32_t jl = j0 + lane;
         const int32_t leaf_l = jl < nn ? p.near_leaf[near0 + jl] : 0;
@@ -676,6 +698,7 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     const bool slots = planned && precision == CAGR_PREC_FP32C;
     CAGR_REQUIRE(!planned || t->plan_slots >= 0, CAGR_ERR_STATE, "no adjoint plan on this tree");
     if (slots) {
+        p.pieces = t->plan_pieces;
         p.wave_off = t->plan_wave_off;
         p.slot_pos = t->plan_slot_pos;
         p.far_val = t->plan_far_val;
@@ -698,8 +721,8 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
 #define CAGR_ADJ32(GMA, CT)                                                                      \
     do {                                                                                         \
         if (slots)                                                                               \
-            k_adj32_list<GMA, CT><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src,                  \
-                                                               t->pts64, t->pts32);              \
+            k_adj32_list<GMA, CT><<<p.pieces ? t->plan_num_pieces : blocks, 64, 0, st>>>(         \
+                p, t->topo, nodes, src, t->pts64, t->pts32);                                     \
         else                                                                                     \
             k_adj32<GMA, CT, FM_ATOMIC><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src,            \
                                                                      t->pts64, t->pts32);        \
@@ -749,6 +772,7 @@ void adjoint_plan_free(cagr_tree* t) {
     (void)hipFree(t->plan_near_off);
     (void)hipFree(t->plan_near_leaf);
     (void)hipFree(t->plan_near_mask);
+    (void)hipFree(t->plan_pieces);
     (void)hipFree(t->plan_nodes);
     (void)hipFree(t->plan_src);
     (void)hipFree(t->plan_c64);
@@ -759,8 +783,67 @@ void adjoint_plan_free(cagr_tree* t) {
     t->plan_wave_off = t->plan_node_off = t->plan_slot_pos = nullptr;
     t->plan_far_node = t->plan_near_off = t->plan_near_leaf = nullptr;
     t->plan_far_mask = t->plan_near_mask = nullptr;
+    t->plan_pieces = nullptr;
+    t->plan_num_pieces = 0;
     t->plan_far_val = nullptr;
     t->plan_slots = -1;
+}
+
+// The replay's pieces: a wave's cost is its far steps plus kNearCost per
+// near entry (a near entry's transposed block costs about as much as that
+// many far steps); a wave costing more than half the mean is cut into ranges
+// of about equal cost (at most 8), far and near lists alike; the pieces are
+// dispatched longest first within each XCD's contiguous share of the slots.
+constexpr int kNearCost = 10;
+int plan_pieces(cagr_tree* t, int64_t nw, hipStream_t st) {
+    std::vector<int32_t> woff(nw + 1), noff(nw + 1);
+    CAGR_HIP(hipMemcpyAsync(woff.data(), t->plan_wave_off, (nw + 1) * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipMemcpyAsync(noff.data(), t->plan_near_off, (nw + 1) * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    std::vector<double> cost(nw);
+    double mean = 0.0;
+    for (int64_t w = 0; w < nw; ++w) {
+        cost[w] = (double)(woff[w + 1] - woff[w]) + kNearCost * (double)(noff[w + 1] - noff[w]);
+        mean += cost[w];
+    }
+    mean = std::max(1.0, mean / (double)std::max<int64_t>(nw, 1));
+    std::vector<int32_t> pcs;
+    std::vector<double> pcost;
+    for (int64_t w = 0; w < nw; ++w) {
+        const int P = (int)std::min(8.0, std::max(1.0, std::ceil(cost[w] / (0.5 * mean))));
+        const int32_t nf = woff[w + 1] - woff[w], nn = noff[w + 1] - noff[w];
+        for (int j = 0; j < P; ++j) {
+            const int32_t pc[8] = {(int32_t)w,
+                                   woff[w] + (int32_t)((int64_t)nf * j / P),
+                                   woff[w] + (int32_t)((int64_t)nf * (j + 1) / P),
+                                   noff[w] + (int32_t)((int64_t)nn * j / P),
+                                   noff[w] + (int32_t)((int64_t)nn * (j + 1) / P), 0, 0, 0};
+            pcs.insert(pcs.end(), pc, pc + 8);
+            pcost.push_back(cost[w] / P);
+        }
+    }
+    const int64_t G = (int64_t)pcost.size();
+    std::vector<int32_t> ord(G);
+    for (int64_t i = 0; i < G; ++i) ord[i] = (int32_t)i;
+    const int64_t q = G / kQueues, r = G % kQueues;
+    int64_t lo = 0;
+    for (int x = 0; x < kQueues; ++x) {
+        const int64_t len = q + (x < r ? 1 : 0);
+        std::stable_sort(ord.begin() + lo, ord.begin() + lo + len,
+                         [&](int32_t a, int32_t b) { return pcost[a] > pcost[b]; });
+        lo += len;
+    }
+    std::vector<int32_t> sorted(8 * G);
+    for (int64_t i = 0; i < G; ++i)
+        for (int k = 0; k < 8; ++k) sorted[8 * i + k] = pcs[8 * (int64_t)ord[i] + k];
+    CAGR_HIP(hipMalloc(&t->plan_pieces, std::max<int64_t>(G, 1) * 8 * sizeof(int32_t)));
+    CAGR_HIP(hipMemcpyAsync(t->plan_pieces, sorted.data(), G * 8 * sizeof(int32_t),
+                            hipMemcpyHostToDevice, st));
+    CAGR_HIP(hipStreamSynchronize(st));
+    t->plan_num_pieces = G;
+    return CAGR_OK;
 }
 
 int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
@@ -857,7 +940,7 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
         CAGR_HIP(hipStreamSynchronize(st));
     }
     t->plan_slots = F;
-    return CAGR_OK;
+    return plan_pieces(t, nw, st);
 }
 
 }  // namespace cagr

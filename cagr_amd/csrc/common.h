@@ -140,6 +140,12 @@ struct cagr_tree {
     int32_t* plan_near_off = nullptr;  // [waves + 1]
     int32_t* plan_near_leaf = nullptr; // [near entries]
     uint64_t* plan_near_mask = nullptr;// [near entries]
+    // the replay's pieces: the far steps and near entries of a long wave cut
+    // into ranges of about equal cost, each range replayed as its own wave
+    // {wave, far first, far end, near first, near end, -, -, -} (null: one
+    // piece per wave, all of its lists), in dispatch order
+    int32_t* plan_pieces = nullptr;
+    int64_t plan_num_pieces = 0;
     int64_t plan_slots = -1;           // -1: no plan
     // FP32 node and source records of the plan's centres (an upward pass of
     // plan_w with no attributes), kept after the first planned call with
