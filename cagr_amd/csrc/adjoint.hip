@@ -52,6 +52,7 @@ struct AP {
     int wdtype;
     double an, sign;
     double open_factor;       // 2 / theta
+    dev::CutBand cb;          // band of the FP32 near-pair cutoff test
     const double* c64;        // FP64 centres of the node records, [m, 3] (retests of the
                               // FP32 opening and cutoff tests)
     double* adj_node;         // [m,3] accumulators L
@@ -170,11 +171,15 @@ __device__ inline int near_query(const NearGroups& g, int b0, bool& ok) {
 
 // The transposed near block of one leaf for the wave's near lanes qmask
 // (group layout above); the groups' sums meet in every group (group_sum).
+// Y64: the FP64 cutoff retests read the tree's FP64 points (false: the
+// records' float coordinates, exact only for a tree of FP32 points, and
+// fewer registers at the retest).
 // One FP64 atomic per source and component into N.  All 64 lanes must be
 // active.
-template <bool GA, bool CONT>
+template <bool GA, bool CONT, bool Y64>
 __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qmask, int lane,
-                                     const SrcV32* __restrict__ src, const double* x,
+                                     const SrcV32* __restrict__ src,
+                                     const double* __restrict__ pts64, const double* x,
                                      const float* xf, const float* h, const float* gin, float w2,
                                      double w, float winv, float ew, float hanf) {
     const NearGroups ng = near_groups(qmask, t.end - t.begin, lane);
@@ -203,15 +208,20 @@ __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qma
             const bool use = has && qok;
             const float d[3] = {qx[0] - sr.y[0], qx[1] - sr.y[1], qx[2] - sr.y[2]};
             const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-            const bool amb = use && fabsf(r2 - qw2) <= 1e-5f * qw2;
+            const bool amb = use && fabsf(r2 - qw2) <= fmaf(p.cb.nb1, qw2, p.cb.nb0);
             bool in = r2 < qw2;
             if (__builtin_expect(__ballot(amb) != 0, 0)) {
                 const double qx64[3] = {pick(x[0]), pick(x[1]), pick(x[2])};
                 const double qw64 = pick(w);
+                double y[3];
+                if constexpr (Y64) {
+                    const double* yp = pts64 + 3 * (int64_t)(has ? j : t.begin);
+                    for (int k = 0; k < 3; ++k) y[k] = yp[k];
+                } else {
+                    for (int k = 0; k < 3; ++k) y[k] = (double)sr.y[k];
+                }
                 if (amb)
-                    in = !(sqrt(dist2_rn(qx64[0] - (double)sr.y[0],
-                                         qx64[1] - (double)sr.y[1],
-                                         qx64[2] - (double)sr.y[2])) >= qw64);
+                    in = !(sqrt(dist2_rn(qx64[0] - y[0], qx64[1] - y[1], qx64[2] - y[2])) >= qw64);
             }
             adj_term32<GA, CONT>(zz, d, r2, use, in, qh, sr.e, hanf, qg, qwinv, qew);
         }
@@ -224,7 +234,7 @@ __device__ inline void adj_near_leaf(const AP& p, const TopoRec& t, uint64_t qma
     }
 }
 
-template <int GMODE_A, bool CONT, int FM>
+template <int GMODE_A, bool CONT, int FM, bool Y64 = false>
 __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restrict__ topo,
                                                     const NodeV32* __restrict__ nodes,
                                                     const SrcV32* __restrict__ src,
@@ -338,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
         // lanes parked past the subtree
         if (topo_leaf(t)) {
             if (MATH && qmask != 0)
-                adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
+                adj_near_leaf<GA, CONT, Y64>(p, t, qmask, lane, src, pts64, x, xf, h, gin, w2, w, winv, ew, hanf);
             if constexpr (FM == FM_COUNT || FM == FM_FILL) {
                 if (qmask != 0) {
                     if constexpr (FM == FM_FILL) {
@@ -370,7 +380,7 @@ __global__ __launch_bounds__(kThreads) void k_adj32(AP p, const TopoRec* __restr
 // the slots; near leaves come from the wave's near list.  With pieces, a
 // wave's lists are cut into ranges replayed by separate waves (the slots
 // and the near atomics do not depend on which wave takes a step).
-template <int GMODE_A, bool CONT>
+template <int GMODE_A, bool CONT, bool Y64>
 __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __restrict__ topo,
                                                          const NodeV32* __restrict__ nodes,
                                                          const SrcV32* __restrict__ src,
@@ -476,7 +486,7 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
             const int32_t n = __builtin_amdgcn_readlane(leaf_l, j);
             const uint64_t qmask = lane_bcast_u64(nmask_l, j);
             const TopoRec t = rec32(topo, n);
-            adj_near_leaf<GA, CONT>(p, t, qmask, lane, src, x, xf, h, gin, w2, w, winv, ew, hanf);
+            adj_near_leaf<GA, CONT, Y64>(p, t, qmask, lane, src, pts64, x, xf, h, gin, w2, w, winv, ew, hanf);
         }
     }
 }
@@ -686,6 +696,7 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     p.an = an;
     p.sign = op == CAGR_OP_A_CPU ? -1.0 : 1.0;
     p.open_factor = t->open_factor;
+    p.cb = dev::cut_band(t->qround);
     p.adj_node = t->adj_node;
     p.adj_pt = t->adj_pt;
     p.kdir = 0;
@@ -718,14 +729,21 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     if (precision == CAGR_PREC_FP32C) {
         auto nodes = (const NodeV32*)nodes_in;
         auto src = (const SrcV32*)src_in;
-#define CAGR_ADJ32(GMA, CT)                                                                      \
+        // trees of FP64 points retest the near pairs' cutoffs with them
+        const bool y64 = t->qround != 0.0;
+#define CAGR_ADJ32_Y(GMA, CT, Y)                                                                 \
     do {                                                                                         \
         if (slots)                                                                               \
-            k_adj32_list<GMA, CT><<<p.pieces ? t->plan_num_pieces : blocks, 64, 0, st>>>(         \
+            k_adj32_list<GMA, CT, Y><<<p.pieces ? t->plan_num_pieces : blocks, 64, 0, st>>>(      \
                 p, t->topo, nodes, src, t->pts64, t->pts32);                                     \
         else                                                                                     \
-            k_adj32<GMA, CT, FM_ATOMIC><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src,            \
-                                                                     t->pts64, t->pts32);        \
+            k_adj32<GMA, CT, FM_ATOMIC, Y><<<blocks, 64, 0, st>>>(p, t->topo, nodes, src,         \
+                                                                    t->pts64, t->pts32);         \
+    } while (0)
+#define CAGR_ADJ32(GMA, CT)                                                                      \
+    do {                                                                                         \
+        if (y64) CAGR_ADJ32_Y(GMA, CT, true);                                                    \
+        else CAGR_ADJ32_Y(GMA, CT, false);                                                       \
     } while (0)
         if (ga) {
             if (continuous) CAGR_ADJ32(1, true);
@@ -735,6 +753,7 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
             else CAGR_ADJ32(0, false);
         }
 #undef CAGR_ADJ32
+#undef CAGR_ADJ32_Y
     } else {
         auto nodes = (const NodeV64*)nodes_in;
         auto src = (const SrcV64*)src_in;
@@ -872,6 +891,7 @@ int adjoint_plan(cagr_tree* t, const void* weights, int dtype, hipStream_t st) {
     p.stop = (int32_t)m;
     p.perm = t->perm;
     p.open_factor = t->open_factor;
+    p.cb = dev::cut_band(t->qround);
     p.wave_cnt = t->plan_wave_off;
     p.near_cnt = t->plan_near_off;
     p.node_cnt = t->plan_node_off;

@@ -90,6 +90,9 @@ struct cagr_tree {
     cagr::TopoRec* topo = nullptr;// [m] preorder
     int32_t* order = nullptr;     // [m] node ids by depth, preorder within a depth
     float4* lbound = nullptr;     // [m] bounding sphere {centre, radius} of each leaf's points
+    // largest |x - (double)(float)x| over the points' coordinates (0 for FP32
+    // points): how far the FP32 walks' float query copies are off
+    double qround = 0.0;
     std::vector<int32_t> depth_off;  // host: order[] offsets of depths 0..16 (size 17)
     // the upward pass: one launch over the leaves (64 consecutive node ids a
     // wave), then one launch per band of at most 3 depths, deepest band

@@ -32,17 +32,25 @@ __device__ inline T sel3(const T* a, int i) {
     return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
 }
 
-// r >= w decided as the FP64 reference would: FP32 unless too close to call
-// (lanes with use = false never take the FP64 retest)
-__device__ inline bool inside_fp32(float r2, float w2, double xd0, double xd1, double xd2,
-                                   double w, bool use = true) {
-    const bool amb = use && fabsf(r2 - w2) <= 1e-5f * w2;
-    bool in = r2 < w2;
-    if (__builtin_expect(__ballot(amb) != 0, 0)) {
-        if (amb) in = !(sqrt(dist2_rn(xd0, xd1, xd2)) >= w);
-    }
-    return in;
+// The band of a near pair's FP32 cutoff test |r2 - w2| <= nb1 w2 + nb0.
+// FP32 arithmetic on the pair's float coordinates is off by far less than
+// 1e-5 w2 near r = w.  When the tree's points are FP64 values rounded to
+// FP32 by up to qr per component (qr = 0 for FP32 points), the query and the
+// source add an error of at most 2 qr per component of d, which moves r2 by
+// at most 4 sqrt(3) r qr + 12 qr^2 <= 4 qr (w2 + 1) + 64 qr^2 where the test
+// can go either way (r within a few qr of w): nb1 = 1e-5 + 4 qr,
+// nb0 = 4 qr + 64 qr^2, each rounded up to a float.
+struct CutBand {
+    float nb1, nb0;
+};
+inline CutBand cut_band(double qr) {
+    const auto up = [](double v) {
+        const float f = (float)v;
+        return (double)f < v ? nextafterf(f, INFINITY) : f;
+    };
+    return CutBand{qr > 0 ? up(1e-5 + 4.0 * qr) : 1e-5f, up(4.0 * qr + 64.0 * qr * qr)};
 }
+
 
 // r >= w for a far node decided as the FP64 reference would, from the FP32
 // distance^2 to the node's float centre: r2 carries the centre's rounding
@@ -242,8 +250,21 @@ __device__ __forceinline__ void loads_here(const SrcS64<K>& a, const SrcS64<K>& 
     }
 }
 
-// f(rec) for the wave-uniform records arr[b..e) in order, two at a time, the
-// two records' loads waited for once
+// r >= w of query x and source y decided as the FP64 reference would: FP32
+// unless within band (nb1 w2 + nb0) of w2, then the FP64 test with the FP64
+// points (y read only there; lanes with use = false never take it)
+__device__ inline bool inside_fp32(float r2, float w2, float band, const double* x,
+                                   const double* y, double w, bool use = true) {
+    const bool amb = use && fabsf(r2 - w2) <= band;
+    bool in = r2 < w2;
+    if (__builtin_expect(__ballot(amb) != 0, 0)) {
+        if (amb) in = !(sqrt(dist2_rn(x[0] - y[0], x[1] - y[1], x[2] - y[2])) >= w);
+    }
+    return in;
+}
+
+// f(rec, index) for the wave-uniform records arr[b..e) in order, two at a
+// time, the two records' loads waited for once
 template <typename R, typename F>
 __device__ __forceinline__ void for_each_rec(const R* __restrict__ arr, int32_t b, int32_t e,
                                              F&& f) {
@@ -253,10 +274,10 @@ __device__ __forceinline__ void for_each_rec(const R* __restrict__ arr, int32_t 
         const R r1 = arr[j + 1];
         loads_here(r0, r1);
         __builtin_amdgcn_sched_barrier(0);
-        f(r0);
-        f(r1);
+        f(r0, j);
+        f(r1, j + 1);
     }
-    if (j < e) f(arr[j]);
+    if (j < e) f(arr[j], j);
 }
 
 // Element i of a record array addressed by a 32-bit byte offset: a scalar

@@ -73,6 +73,7 @@ struct TP {
     int64_t* cnt_visit;
     double open_factor;       // 2 / theta
     const float4* lbound;     // leaf bounding spheres
+    dev::CutBand cb;          // band of the FP32 near-pair cutoff test
     const int32_t* tile_order;// regular grid: dispatch slot -> wave (null: identity)
     const double* r64;        // FP64 centres of the upward pass, [set][node][3] (retests)
     int64_t nm;               // nodes (set stride of r64)
@@ -262,7 +263,10 @@ __device__ inline void finish_v32(const TP& p, const Sink& sk, const VAcc32<VOP,
 // root only if its own walk reaches root, and otherwise resumes after the
 // shallowest ancestor it takes far -- the piece then takes exactly the steps
 // of the whole walk that fall in [root, stop).  Results go to sk.
-template <int P, int VOP, int QM, bool CONT, bool CNT>
+// Y64: the near pairs' FP64 cutoff retests read the tree's FP64 points
+// (false: the source records' float coordinates, exact only for a tree of
+// FP32 points, but it keeps the points' address out of the walk's registers)
+template <int P, int VOP, int QM, bool CONT, bool CNT, bool Y64 = true>
 __device__ __forceinline__ void trav_v(
     const TP& p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
@@ -296,6 +300,7 @@ __device__ __forceinline__ void trav_v(
         }
         const float wf = (float)w;
         const float w2 = wf * wf;
+        const float cband = fmaf(p.cb.nb1, w2, p.cb.nb0);   // near pairs' cutoff band
         const int dsafe = __builtin_amdgcn_readfirstlane(
             far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
         if (chain) {   // the ancestors' opening tests, exactly as the steps below take them
@@ -409,7 +414,7 @@ __device__ __forceinline__ void trav_v(
                         if (nearl) acc.flush_leaf();
                     } else if (nearl) {
                         if (fast) {   // no pair of this leaf is within any lane's width
-                            for_each_rec(src, t.begin, t.end, [&](const SrcV32& sr) {
+                            for_each_rec(src, t.begin, t.end, [&](const SrcV32& sr, int32_t) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
@@ -417,14 +422,17 @@ __device__ __forceinline__ void trav_v(
                                 acc.add(acc.lp, d, r2, false, sr.mu, sr.e, sr.q);
                             });
                         } else {
-                            for_each_rec(src, t.begin, t.end, [&](const SrcV32& sr) {
+                            for_each_rec(src, t.begin, t.end, [&](const SrcV32& sr, int32_t j) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
                                 const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                                const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
-                                                            x[1] - (double)sr.y[1],
-                                                            x[2] - (double)sr.y[2], w);
+                                // the FP64 source: the tree's FP64 point, or (FP32
+                                // points only) the record's own coordinates
+                                double yr[3];
+                                for (int k = 0; k < 3; ++k) yr[k] = (double)sr.y[k];
+                                const bool in = inside_fp32(r2, w2, cband, x,
+                                                            Y64 ? pts64 + 3 * (int64_t)j : yr, w);
                                 acc.add(acc.lp, d, r2, in, sr.mu, sr.e, sr.q);
                             });
                         }
@@ -515,7 +523,7 @@ __device__ __forceinline__ void trav_v(
                             }
                         }
                     } else if (nearl) {
-                        for_each_rec(src, t.begin, t.end, [&](const SrcV64& sr) {
+                        for_each_rec(src, t.begin, t.end, [&](const SrcV64& sr, int32_t) {
                             const double dd[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
                             acc.add(dd, w, sr.mu, p.an);
                         });
@@ -570,7 +578,7 @@ __device__ __forceinline__ void trav_v(
 // scalar family: AT (an = 0), A2Tk (K = 1, direction kdir), sum_k A2Tk (K = 3)
 // ============================================================================
 // (pieces: as trav_v, one resume per set)
-template <int P, int K, int QM, bool CNT>
+template <int P, int K, int QM, bool CNT, bool Y64 = true>
 __device__ __forceinline__ void trav_s(
     const TP& p, const TopoRec* __restrict__ topo, const void* __restrict__ nodes_,
     const void* __restrict__ src_, const double* __restrict__ pts64,
@@ -607,6 +615,7 @@ __device__ __forceinline__ void trav_s(
         const float hanf = (float)han;
         const float wf = (float)w;
         const float w2 = wf * wf;
+        const float cband = fmaf(p.cb.nb1, w2, p.cb.nb0);   // near pairs' cutoff band
         const int dsafe = __builtin_amdgcn_readfirstlane(
             far_safe_depth(p.open_factor, wave_max(valid ? w : 0.0)));
         if (chain) {   // the ancestors' opening tests of every set (see trav_v)
@@ -828,7 +837,7 @@ __device__ __forceinline__ void trav_s(
 #pragma unroll
                         for (int k = 0; k < K; ++k) mk[k] = nearl[k] ? eq[k] : 0.f;
                         if (fast) {   // no pair of this leaf is within any lane's width
-                            for_each_rec(src, t.begin, t.end, [&](const SrcS32<K>& sr) {
+                            for_each_rec(src, t.begin, t.end, [&](const SrcS32<K>& sr, int32_t) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
@@ -836,14 +845,17 @@ __device__ __forceinline__ void trav_s(
                                 near_pair(lp, sr, d, r2, false, mk);
                             });
                         } else {
-                            for_each_rec(src, t.begin, t.end, [&](const SrcS32<K>& sr) {
+                            for_each_rec(src, t.begin, t.end, [&](const SrcS32<K>& sr, int32_t j) {
                                 float d[3] = {xf[0] - sr.y[0], xf[1] - sr.y[1], xf[2] - sr.y[2]};
                                 if constexpr (QM != QM_SELF)
                                     for (int k = 0; k < 3; ++k) d[k] += xl[k];
                                 const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-                                const bool in = inside_fp32(r2, w2, x[0] - (double)sr.y[0],
-                                                            x[1] - (double)sr.y[1],
-                                                            x[2] - (double)sr.y[2], w);
+                                // the FP64 source: the tree's FP64 point, or (FP32
+                                // points only) the record's own coordinates
+                                double yr[3];
+                                for (int k = 0; k < 3; ++k) yr[k] = (double)sr.y[k];
+                                const bool in = inside_fp32(r2, w2, cband, x,
+                                                            Y64 ? pts64 + 3 * (int64_t)j : yr, w);
                                 near_pair(lp, sr, d, r2, in, mk);
                             });
                         }
@@ -980,7 +992,7 @@ __device__ __forceinline__ void trav_s(
                             if (any_near) acc[cc] += v;
                         }
                     } else if (any_near) {
-                        for_each_rec(src, t.begin, t.end, [&](const SrcS64<K>& sr) {
+                        for_each_rec(src, t.begin, t.end, [&](const SrcS64<K>& sr, int32_t) {
                             const double d[3] = {x[0] - sr.y[0], x[1] - sr.y[1], x[2] - sr.y[2]};
                             for (int k = 0; k < K; ++k)
                                 if (nearl[k]) add(d, sr.s[k], p.kdir + k);
@@ -1073,8 +1085,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_trav_v32(
     const int64_t slot =
         xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const WaveWork ww = wave_work(p, slot);
-    trav_v<CAGR_PREC_FP32C, VOP, QM, CONT, false>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq,
-                                                  ww.lo, ww.hi, ww.part, ww.chain);
+    trav_v<CAGR_PREC_FP32C, VOP, QM, CONT, false, false>(p, topo, nodes_, src_, pts64, pts32, ww.q0,
+                                                         p.nq, ww.lo, ww.hi, ww.part, ww.chain);
 }
 
 template <int K, int QM>
@@ -1085,8 +1097,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_trav_s32(
     const int64_t slot =
         xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const WaveWork ww = wave_work(p, slot);
-    trav_s<CAGR_PREC_FP32C, K, QM, false>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq, ww.lo,
-                                          ww.hi, ww.part, ww.chain);
+    trav_s<CAGR_PREC_FP32C, K, QM, false, false>(p, topo, nodes_, src_, pts64, pts32, ww.q0, p.nq,
+                                                 ww.lo, ww.hi, ww.part, ww.chain);
 }
 
 // Persistent form: a grid sized to the device pulls wave items {root, stop,
@@ -1153,8 +1165,9 @@ __global__ __launch_bounds__(512, 8) void k_trav_v_items32(
     const float* __restrict__ pts32, Items q) {
     int4 it;
     while (next_item(q, it))
-        trav_v<CAGR_PREC_FP32C, VOP, QM_SELF, false, false>(p, topo, nodes_, src_, pts64, pts32,
-                                                           it.z, it.z + it.w, it.x, it.y);
+        trav_v<CAGR_PREC_FP32C, VOP, QM_SELF, false, false, false>(p, topo, nodes_, src_, pts64,
+                                                                  pts32, it.z, it.z + it.w, it.x,
+                                                                  it.y);
 }
 
 template <int K>
@@ -1164,8 +1177,8 @@ __global__ __launch_bounds__(512, 8) void k_trav_s_items32(
     const float* __restrict__ pts32, Items q) {
     int4 it;
     while (next_item(q, it))
-        trav_s<CAGR_PREC_FP32C, K, QM_SELF, false>(p, topo, nodes_, src_, pts64, pts32, it.z,
-                                                   it.z + it.w, it.x, it.y);
+        trav_s<CAGR_PREC_FP32C, K, QM_SELF, false, false>(p, topo, nodes_, src_, pts64, pts32,
+                                                          it.z, it.z + it.w, it.x, it.y);
 }
 
 __global__ void k_reset_counter(int32_t* c) {
@@ -1247,7 +1260,7 @@ __global__ __launch_bounds__(64) void k_walk_cost(const TopoRec* __restrict__ to
 template <int P, int VOP, int QM, bool CNT>
 int launch_v(const TP& tp, cagr_tree* t, int continuous, int64_t blocks, hipStream_t st) {
     if constexpr (P == CAGR_PREC_FP32C && !CNT && QM == QM_SELF) {
-        if (!continuous) {
+        if (!continuous && t->qround == 0.0) {
             k_trav_v32<VOP, QM, false><<<blocks, 64 * t->grid_waves, 0, st>>>(
                 tp, t->topo, t->nodes, t->src, t->pts64, t->pts32);
             CAGR_LAUNCHED();
@@ -1277,12 +1290,16 @@ int launch_vop(const TP& tp, cagr_tree* t, int vop, int continuous, int64_t bloc
 
 template <int P, int K, int QM, bool CNT>
 int launch_s(const TP& tp, cagr_tree* t, int64_t blocks, hipStream_t st) {
-    if constexpr (P == CAGR_PREC_FP32C && !CNT && QM == QM_SELF)
-        k_trav_s32<K, QM><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes, t->src,
-                                                                 t->pts64, t->pts32);
-    else
-        k_trav_s<P, K, QM, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes,
-                                                                       t->src, t->pts64, t->pts32);
+    if constexpr (P == CAGR_PREC_FP32C && !CNT && QM == QM_SELF) {
+        if (t->qround == 0.0) {   // the 64-VGPR walk: a tree of FP32 points only
+            k_trav_s32<K, QM><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes, t->src,
+                                                                     t->pts64, t->pts32);
+            CAGR_LAUNCHED();
+            return CAGR_OK;
+        }
+    }
+    k_trav_s<P, K, QM, CNT><<<blocks, 64 * t->grid_waves, 0, st>>>(tp, t->topo, t->nodes, t->src,
+                                                                   t->pts64, t->pts32);
     CAGR_LAUNCHED();
     return CAGR_OK;
 }
@@ -1300,6 +1317,9 @@ int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t
     const auto* topo = t->topo;
     const void* nodes = t->nodes;
     const void* src = t->src;
+    // the 64-VGPR walks only for trees of FP32 points (their cutoff retests
+    // take the source records' coordinates)
+    const bool fp32_tree = t->qround == 0.0;
     switch (op) {
         case CAGR_OP_A:
         case CAGR_OP_A_CPU:
@@ -1317,7 +1337,7 @@ int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t
         case CAGR_OP_A2:
             if (continuous)
                 k_trav_v_items<P, VOP_A2, true, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
-            else if constexpr (P == CAGR_PREC_FP32C && !CNT)
+            else if (fp32_tree && P == CAGR_PREC_FP32C && !CNT)
                 k_trav_v_items32<VOP_A2><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             else
                 k_trav_v_items<P, VOP_A2, false, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
@@ -1335,7 +1355,7 @@ int launch_items(const TP& tp, cagr_tree* t, int op, int continuous, hipStream_t
             k_trav_s_items<P, 1, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             break;
         case CAGR_OP_A2T_SUM:
-            if constexpr (P == CAGR_PREC_FP32C && !CNT)
+            if (fp32_tree && P == CAGR_PREC_FP32C && !CNT)
                 k_trav_s_items32<3><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
             else
                 k_trav_s_items<P, 3, CNT><<<blocks, threads, 0, st>>>(tp, topo, nodes, src, t->pts64, t->pts32, q);
@@ -1403,6 +1423,7 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
     tp.cnt_visit = cnt_visit;
     tp.open_factor = t->open_factor;
     tp.lbound = t->lbound;
+    tp.cb = dev::cut_band(t->qround);
     tp.r64 = t->r64;
     tp.nm = t->m;
     int64_t lanes;

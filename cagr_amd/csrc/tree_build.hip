@@ -256,8 +256,31 @@ __global__ __launch_bounds__(kThreads) void k_leaf_of(const TopoRec* __restrict_
 // bounding sphere of every leaf's points: centre = middle of their bounding
 // box (rounded to float), radius = the largest FP64 distance to that centre,
 // rounded up; used only to prove that a whole leaf lies outside a width sphere
+// Largest |p64 - (double)p32| over the coordinates into *out (as the bits
+// of a non-negative double, which order like the values; NaN never wins):
+// how far the float copies the FP32 walks take as queries are off
+__global__ __launch_bounds__(kThreads) void k_round_err(const double* __restrict__ p64,
+                                                        const float* __restrict__ p32, int64_t len,
+                                                        unsigned long long* __restrict__ out) {
+    double e = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < len;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double v = fabs(p64[i] - (double)p32[i]);
+        if (v > e) e = v;
+    }
+    for (int s = 32; s > 0; s >>= 1) {
+        const double o = __shfl_xor(e, s, 64);
+        if (o > e) e = o;
+    }
+    if ((threadIdx.x & 63) == 0 && e > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
+}
+
+// bounding sphere of each leaf's FP64 points, its radius widened by 2 qr
+// (the FP32 walks test it with a float query copy off by up to qr per
+// component)
 __global__ __launch_bounds__(kThreads) void k_leaf_bounds(const TopoRec* __restrict__ topo,
                                                           int64_t m, const double* __restrict__ p64,
+                                                          const unsigned long long* __restrict__ qbits,
                                                           float4* __restrict__ lb) {
     const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= m) return;
@@ -283,7 +306,8 @@ __global__ __launch_bounds__(kThreads) void k_leaf_bounds(const TopoRec* __restr
         }
         r2 = fmax(r2, q);
     }
-    const float r = (float)(sqrt(r2) * (1.0 + 1e-6)) + 1e-7f;
+    const double qr = __longlong_as_double((long long)*qbits);
+    const float r = (float)((sqrt(r2) + 2.0 * qr) * (1.0 + 1e-6)) + 1e-7f;
     // a leaf with a non-finite point never qualifies
     lb[i] = make_float4(c[0], c[1], c[2], (r2 == r2 && r2 < 1e300) ? r : 3.0e38f);
 }
@@ -672,9 +696,22 @@ int build_tree(cagr_tree* t, const void* points, int64_t n, int dtype, int max_d
     k_leaf_of<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->leaf_of);
     CAGR_LAUNCHED();
     CAGR_HIP(hipMalloc(&t->lbound, m * sizeof(float4)));
-    k_leaf_bounds<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->pts64, t->lbound);
+    unsigned long long* qbits = nullptr;
+    CAGR_HIP(hipMalloc(&qbits, sizeof(unsigned long long)));
+    struct FreeQ {
+        unsigned long long* p;
+        ~FreeQ() { (void)hipFree(p); }
+    } free_q{qbits};
+    CAGR_HIP(hipMemsetAsync(qbits, 0, sizeof(unsigned long long), st));
+    k_round_err<<<(unsigned)std::min<int64_t>(1024, blocks_for(3 * n)), kThreads, 0, st>>>(
+        t->pts64, t->pts32, 3 * n, qbits);
     CAGR_LAUNCHED();
+    k_leaf_bounds<<<blocks_for(m), kThreads, 0, st>>>(t->topo, m, t->pts64, qbits, t->lbound);
+    CAGR_LAUNCHED();
+    unsigned long long qb = 0;
+    CAGR_HIP(hipMemcpyAsync(&qb, qbits, sizeof(qb), hipMemcpyDeviceToHost, st));
     CAGR_HIP(hipStreamSynchronize(st));
+    t->qround = __builtin_bit_cast(double, qb);
     return CAGR_OK;
 }
 

@@ -52,20 +52,23 @@ __device__ inline float down_float(double x) {
     return (double)f > x ? nextafterf(f, -INFINITY) : f;
 }
 
-// The FP32 scalar-family walk decides far/near on the float centre cf =
-// (float)c of each set (NodeS32): it forms
+// The FP32 walks decide far/near on the float centre cf = (float)c of each
+// set (NodeV32, NodeS32): they form
 // d = (x_f - cf) (+ x_lo, the low part of an FP64 query split into floats)
-// and r2 = fma(d0, d0, fma(d1, d1, d2 * d2)).  Per component d is then off by
-// at most E = |c - cf| + 2^-23 (|d| + |c - cf|) + 2^-46 (|c| + 2 |d|) (two
-// FP32 roundings and the split of the query), so at |d| = thr
+// and r2 = fma(d0, d0, fma(d1, d1, d2 * d2)).  A query point of the tree
+// itself is taken as its float copy x_f, off by at most qr per component
+// from the FP64 point (0 when the points are FP32 values).  Per component d
+// is then off by at most E = |c - cf| + qr + 2^-23 (|d| + |c - cf| + qr) +
+// 2^-46 (|c| + 2 |d|) (two FP32 roundings and the split of the query), so at
+// |d| = thr
 // |r2 - dist^2| <= B = 2 sqrt(3) thr E + 3 E^2 + 2^-22 thr^2 (r2's own
 // roundings) + 2^-50 thr^2 (the FP64 reference's rounding of its dist^2).
 // The error grows more slowly than |dist^2 - thr^2| away from the threshold,
 // so r2 > hi2 = thr^2 + 2B proves the FP64 test far and r2 < lo2 = thr^2 - 2B
-// proves it near; lanes between them retest in FP64.  ce = |c - cf| +
-// 2^-46 (|c| + 2) is the centre's part of E, which the walks' cutoff test
-// (r >= w) widens its own band by.
-__device__ inline void open_band(const double* c, double thr2, float* cf, float& hi2,
+// proves it near; lanes between them retest in FP64.  ce = |c - cf| + qr +
+// 2^-46 (|c| + 2) is the part of E that is not FP32 arithmetic, which the
+// walks' cutoff test (r >= w) widens its own band by.
+__device__ inline void open_band(const double* c, double thr2, double qr, float* cf, float& hi2,
                                  float& lo2, float& ce) {
     double delta = 0.0, cmax = 0.0;
     for (int d = 0; d < 3; ++d) {
@@ -74,11 +77,11 @@ __device__ inline void open_band(const double* c, double thr2, float* cf, float&
         cmax = fmax(cmax, fabs(c[d]));
     }
     const double thr = sqrt(thr2);
-    const double E = delta + 0x1p-23 * (thr + delta) + 0x1p-46 * (cmax + 2.0 * thr);
+    const double E = delta + qr + 0x1p-23 * (thr + delta + qr) + 0x1p-46 * (cmax + 2.0 * thr);
     const double B = 3.4641016151377546 * thr * E + 3.0 * E * E + (0x1p-22 + 0x1p-50) * thr2;
     hi2 = up_float(thr2 + 2.0 * B);
     lo2 = down_float(thr2 - 2.0 * B);
-    ce = up_float(delta + 0x1p-46 * (cmax + 2.0));
+    ce = up_float(delta + qr + 0x1p-46 * (cmax + 2.0));
 }
 
 // record writers --------------------------------------------------------------
@@ -90,14 +93,14 @@ template <int P>
 struct Rec<0, 1, P> {
     static constexpr int C = 3;
     __device__ static void node(void* nodes, int64_t n, int k, const double* c, const double* mm,
-                                double an, int /*dir0*/, double thr2) {
+                                double an, int /*dir0*/, double thr2, double qr) {
         if constexpr (P == CAGR_PREC_FP64) {
             NodeV64 r;
             for (int d = 0; d < 3; ++d) { r.c[d] = c[d]; r.m[d] = mm[d]; }
             ((NodeV64*)nodes)[n] = r;
         } else {
             NodeV32 r;
-            open_band(c, thr2, r.cf, r.hi2, r.lo2, r.ce);
+            open_band(c, thr2, qr, r.cf, r.hi2, r.lo2, r.ce);
             for (int d = 0; d < 3; ++d) {
                 const double e = exp(-0.5 * an * c[d]);
                 r.m[d] = (float)mm[d];
@@ -136,7 +139,7 @@ template <int K, int P>
 struct Rec<1, K, P> {
     static constexpr int C = 1;
     __device__ static void node(void* nodes, int64_t n, int k, const double* c, const double* mm,
-                                double an, int dir0, double thr2) {
+                                double an, int dir0, double thr2, double qr) {
         if constexpr (P == CAGR_PREC_FP64) {
             NodeS64 r;
             for (int d = 0; d < 3; ++d) r.c[d] = c[d];
@@ -144,7 +147,7 @@ struct Rec<1, K, P> {
             ((NodeS64*)nodes)[n * K + k] = r;
         } else {
             NodeS32 r;
-            open_band(c, thr2, r.cf, r.hi2, r.lo2, r.ce);
+            open_band(c, thr2, qr, r.cf, r.hi2, r.lo2, r.ce);
             r.se = (float)(mm[0] * exp(-0.5 * an * pick3(c, dir0 + k)));
             r.pad = 0.f;
             ((NodeS32*)nodes)[n * K + k] = r;
@@ -193,7 +196,8 @@ __device__ inline void leaf_finish(int32_t nd, const TopoRec& t, const double (&
                                    const double (&acc)[K][Rec<FAM, K, P>::C],
                                    const double (&rep0)[3], int64_t m, double* __restrict__ w64,
                                    double* __restrict__ r64, double* __restrict__ m64,
-                                   void* __restrict__ nodes, double an, int dir0) {
+                                   void* __restrict__ nodes, double an, int dir0,
+                                   double qr) {
 #pragma clang fp contract(off)
     constexpr int C = Rec<FAM, K, P>::C;
     const double np = (double)(t.end - t.begin);
@@ -210,7 +214,7 @@ __device__ inline void leaf_finish(int32_t nd, const TopoRec& t, const double (&
         w64[o] = tw[k];
         for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
         for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[k][cc];
-        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2);
+        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2, qr);
     }
 }
 
@@ -229,7 +233,8 @@ template <int FAM, int K, int P>
 __device__ inline void up_internal_node(int32_t nd, const TopoRec& t, const int32_t ch[8],
                                         int64_t m, double* __restrict__ w64,
                                         double* __restrict__ r64, double* __restrict__ m64,
-                                        void* __restrict__ nodes, double an, int dir0) {
+                                        void* __restrict__ nodes, double an, int dir0,
+                                   double qr) {
 #pragma clang fp contract(off)
     constexpr int C = Rec<FAM, K, P>::C;
     const int nk = topo_nkids(t);
@@ -269,7 +274,7 @@ __device__ inline void up_internal_node(int32_t nd, const TopoRec& t, const int3
         w64[o] = tw[k];
         for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
         for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[k][cc];
-        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2);
+        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2, qr);
     }
 }
 
@@ -285,7 +290,7 @@ __global__ __launch_bounds__(64, 2) void k_up_leaves(
     const double* __restrict__ pts64, const float* __restrict__ pts32,
     const TI* __restrict__ attrs, const void* __restrict__ weights, int wdtype,
     double* __restrict__ w64, double* __restrict__ r64, double* __restrict__ m64,
-    void* __restrict__ nodes, void* __restrict__ src, double an, int dir0) {
+    void* __restrict__ nodes, void* __restrict__ src, double an, int dir0, double qr) {
 #pragma clang fp contract(off)
     constexpr int C = Rec<FAM, K, P>::C;
     constexpr int KC = K * C;
@@ -361,7 +366,7 @@ __global__ __launch_bounds__(64, 2) void k_up_leaves(
     }
     if (leaf)
         leaf_finish<FAM, K, P>((int32_t)nd, t, tw, rep, acc, rep0, m, w64, r64, m64, nodes, an,
-                               dir0);
+                               dir0, qr);
 }
 
 // Band launch, one wave per node at the band's top depth: the internal nodes
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(64) void k_up_band(const int2* __restrict__ seg, in
                                                 const TopoRec* __restrict__ topo, int64_t m,
                                                 double* __restrict__ w64, double* __restrict__ r64,
                                                 double* __restrict__ m64, void* __restrict__ nodes,
-                                                double an, int dir0) {
+                                                double an, int dir0, double qr) {
     const int32_t lane = (int32_t)threadIdx.x;
     auto fetch = [&](int l, int32_t& nd, TopoRec& t, int32_t ch[8]) {
         const int2 sg = seg[(int64_t)blockIdx.x * nlev + l];
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(64) void k_up_band(const int2* __restrict__ seg, in
         int32_t nd2 = -1, ch2[8];
         TopoRec t2;
         if (l > 0) fetch(l - 1, nd2, t2, ch2);
-        if (nd >= 0) up_internal_node<FAM, K, P>(nd, t, ch, m, w64, r64, m64, nodes, an, dir0);
+        if (nd >= 0) up_internal_node<FAM, K, P>(nd, t, ch, m, w64, r64, m64, nodes, an, dir0, qr);
         __syncthreads();
         nd = nd2;
         t = t2;
@@ -415,11 +420,11 @@ int run_upward(cagr_tree* t, const TI* attrs, const void* weights, int wdtype, d
     CAGR_REQUIRE(ok, CAGR_ERR_STATE, "upward: bad band");
     k_up_leaves<TI, FAM, K, P><<<(unsigned)((m + 63) / 64), 64, 0, st>>>(
         m, t->topo, t->perm, t->pts64, t->pts32, attrs, weights, wdtype, t->w64, t->r64, t->m64,
-        t->nodes, t->src, an, dir0);
+        t->nodes, t->src, an, dir0, t->qround);
     CAGR_LAUNCHED();
     for (const int4& b : t->up_bands) {
         k_up_band<FAM, K, P><<<b.z, 64, 0, st>>>(t->up_seg + b.w, b.y, t->order, t->topo, m, t->w64,
-                                                 t->r64, t->m64, t->nodes, an, dir0);
+                                                 t->r64, t->m64, t->nodes, an, dir0, t->qround);
         CAGR_LAUNCHED();
     }
     return CAGR_OK;
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(kThreads) void k_nodes_from_moments(int64_t m,
                                                                 const TI* __restrict__ rep,
                                                                 const TopoRec* __restrict__ topo,
                                                                 void* nodes, double* r64,
-                                                                double an, int dir0) {
+                                                                double an, int dir0, double qr) {
     using R = Rec<FAM, 1, P>;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m) return;
@@ -459,7 +464,7 @@ __global__ __launch_bounds__(kThreads) void k_nodes_from_moments(int64_t m,
         r64[3 * n + d] = c[d];
     }
     for (int d = 0; d < R::C; ++d) mm[d] = (double)attrs[n * R::C + d];
-    R::node(nodes, n, 0, c, mm, an, dir0, topo[n].thr2);
+    R::node(nodes, n, 0, c, mm, an, dir0, topo[n].thr2, qr);
 }
 
 template <typename TI>
@@ -472,17 +477,17 @@ int nodes_from_moments_t(cagr_tree* t, int family, int precision, int dir0, cons
     if (family == 0) {
         if (p64)
             k_nodes_from_moments<TI, 0, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(
-                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0, t->qround);
         else
             k_nodes_from_moments<TI, 0, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(
-                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0, t->qround);
     } else {
         if (p64)
             k_nodes_from_moments<TI, 1, CAGR_PREC_FP64><<<nb, kThreads, 0, st>>>(
-                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0, t->qround);
         else
             k_nodes_from_moments<TI, 1, CAGR_PREC_FP32C><<<nb, kThreads, 0, st>>>(
-                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0);
+                m, attrs, rep, t->topo, t->nodes, t->r64, an, dir0, t->qround);
     }
     CAGR_LAUNCHED();
     return CAGR_OK;

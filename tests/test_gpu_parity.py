@@ -10,7 +10,10 @@ Bars (each with its reason):
   decisions, different summation order and 1/r evaluation: observed ~2e-15);
 * FP32-compensated operators: max |gpu - oracle| <= 2e-6 * max |oracle|
   (FP32 kernel sums with compensated accumulation: observed <= 6.3e-7);
-* results returned in float32 carry one more float32 rounding (<= 1e-7).
+* results returned in float32 carry one more float32 rounding (<= 1e-7);
+* FP32c on FP64 points that are not FP32 values, in pairs 5e-4 apart with
+  widths on the pairs' distances: 1e-3 (the float copies' own rounding; a
+  wrong cutoff decision shows as 0.01-1).
 """
 import os
 
@@ -26,6 +29,7 @@ DEV = "cuda:0"
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 TOL64 = 1e-12
 TOL32C = 2e-6
+TOLF64PTS = 1e-3   # FP32c on FP64 points in close pairs (see the test)
 
 
 def say(*a):
@@ -374,6 +378,60 @@ def test_orient_end_to_end_reproduces_bunny(tmp_path, precision):
     assert np.median(cos) > 0.9999 and (cos > 0).mean() == 1.0 and abs(np.median(mag) - 1) < 0.02
     pts, nrm2 = pcio.load_points(os.path.join(out_dir, "bunny.xyz"))
     assert np.array_equal(pts, d["points"]) and np.array_equal(nrm2, nrm.astype(np.float64))
+
+
+@pytest.mark.parametrize("leaf", [1, 16])
+def test_fp64_points_on_the_cutoff_match_oracle(leaf):
+    """FP64 points that are not FP32 values, in close pairs 5e-4 to 1e-3
+    apart, each point's width its partner's FP64 distance times 1 + d with
+    |d| <= 2e-5.  The float copies of the points move a pair's distance by up
+    to ~1e-4 of itself, so FP32 arithmetic alone would decide many of these
+    cutoff tests wrongly, and a wrong decision adds or drops the row's
+    largest term (the kernels are zero inside the width).  The FP32c results
+    match the oracle only if every such test is redone in FP64 on the FP64
+    points: the near pairs (leaf 16) and the single-point far nodes (leaf 1),
+    in the forward operators, the fused forms and both adjoints.
+
+    Bar 1e-3 normwise: a wrong decision moves a row by about its largest term
+    (errors ~0.01-1 of the largest entry), while the FP32c form computes d
+    from the float copies, which moves the dominant terms by up to
+    3 * 2 * 2^-25 / 5e-4 ~ 4e-4 of themselves (measured 1.0e-4)."""
+    say("fp64 cutoff", leaf)
+    rng = np.random.default_rng(31)
+    m = 1500
+    base = synth.normalize(synth.ellipsoids(m, 12))
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    p = np.concatenate([base, base + u * rng.uniform(5e-4, 1e-3, m)[:, None]])
+    assert not np.array_equal(p, p.astype(np.float32).astype(np.float64))
+    n = 2 * m
+    d = p - p[np.concatenate([np.arange(m, n), np.arange(m)])]
+    widths = np.sqrt((d[:, 0] ** 2 + d[:, 1] ** 2) + d[:, 2] ** 2) * (
+        1.0 + rng.uniform(-2e-5, 2e-5, n))
+    wt = WindingNumberTreecode(t(p), leaf_size=leaf)
+    ot = oracle.build_tree(p, 15, leaf)
+    mu = synth.random_normals(n, 5)
+    s3 = rng.normal(size=(n, 3))
+    for op in ("A", "A21", "A2T1", "AT"):
+        a = mu if oracle.IN_DIM[op] == 3 else s3[:, :1]
+        ref = oracle.apply(op, ot, p, a, widths)
+        out = wt.apply(op, t(a), t(widths), precision="fp32c").cpu().numpy()
+        assert normwise(out, ref) <= TOLF64PTS, (op, normwise(out, ref))
+    ref_a = np.concatenate([oracle.apply(f"A2{k}", ot, p, mu, widths) for k in (1, 2, 3)], 1)
+    ref_t = sum(oracle.apply(f"A2T{k + 1}", ot, p, s3[:, k:k + 1], widths) for k in range(3))
+    a = wt.forward_A2(t(mu), t(widths), precision="fp32c").cpu().numpy()
+    r = wt.forward_A2T_sum(t(s3), t(widths), precision="fp32c").cpu().numpy()
+    assert normwise(a, ref_a) <= TOLF64PTS, normwise(a, ref_a)
+    assert normwise(r, ref_t) <= TOLF64PTS, normwise(r, ref_t)
+    wfix = rng.uniform(0.5, 1.5, n)
+    _, rep, _ = oracle.upward(ot, p, np.zeros((n, 3)), wfix)
+    g = rng.normal(size=(n, 1))
+    ref = oracle.adjoint("A", ot, p, g, widths, rep)
+    out = wt.apply_adjoint("A", t(g), t(widths), t(wfix), precision="fp32c").cpu().numpy()
+    assert normwise(out, ref) <= TOLF64PTS, normwise(out, ref)
+    wt.adjoint_plan(t(wfix))
+    out = wt.apply_adjoint_planned("A", t(g), t(widths), precision="fp32c").cpu().numpy()
+    assert normwise(out, ref) <= TOLF64PTS, normwise(out, ref)
 
 
 @pytest.mark.parametrize("leaf", [16, 32, 64])
