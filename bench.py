@@ -371,7 +371,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "1M points on 6 seeded ellipsoids (sigma 5e-4), A.mu + A^T.r matvec pair, "
-                        "FP32 compensated, FP64 opening tests",
+                        "FP32 interactions with FP64 totals, the FP64 operator's far/near and "
+                        "cutoff decisions (FP32 tests, FP64 retests in bounds)",
             "points": n, "leaf_size": args.leaf, "theta": args.theta, "width": args.width,
             "nodes": info["num_nodes"], "depth": info["depth"],
         },

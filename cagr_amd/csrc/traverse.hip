@@ -1189,19 +1189,34 @@ __global__ void k_reset_counter(int32_t* c) {
 // The output of every split wave {wave, first slot, pieces}: the sum of its
 // pieces' partial slots in piece order (a fixed order, so the result does not
 // depend on when the pieces ran), no values per query.  One wave per split
-// wave.
-__global__ __launch_bounds__(64) void k_piece_combine(TP p, const int4* __restrict__ split,
-                                                      int no) {
-    const int4 sw = split[blockIdx.x];
-    const int lane = threadIdx.x;
+// wave, four to a block; a wave's partials are loaded 8 slots at a time before
+// any is added (one memory round trip per 8 pieces, not one per piece).
+constexpr int kCombineWaves = 4;
+__global__ __launch_bounds__(64 * kCombineWaves) void k_piece_combine(
+    TP p, const int4* __restrict__ split, int64_t nsplit, int no) {
+    const int64_t ws = (int64_t)blockIdx.x * kCombineWaves + (threadIdx.x >> 6);
+    if (ws >= nsplit) return;   // whole waves
+    const int4 sw = split[ws];
+    const int lane = threadIdx.x & 63;
     const int64_t s = (int64_t)sw.x * 64 + lane;
     if (s >= p.nq) return;
     const int64_t oi = p.perm[s];
-    for (int k = 0; k < no; ++k) {
-        double v = 0.0;
-        for (int j = 0; j < sw.z; ++j) v += p.planes[((int64_t)(sw.y + j) * 64 + lane) * 3 + k];
-        store_out(p, no * oi + k, v);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < sw.z; j0 += 8) {
+        double v[8][3];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double* pl = p.planes + ((int64_t)(sw.y + j0 + j) * 64 + lane) * 3;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[j][k] = (j0 + j < sw.z && k < no) ? pl[k] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (j0 + j < sw.z) acc[k] += v[j][k];
     }
+    for (int k = 0; k < no; ++k) store_out(p, no * oi + k, acc[k]);
 }
 
 // Cost of the steps of each wave's self-query walk (the decisions of the FP64
@@ -1471,7 +1486,8 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
             const int no = (op == CAGR_OP_G || op == CAGR_OP_A2 || op == CAGR_OP_AT ||
                             op == CAGR_OP_A2T_SUM || (op >= CAGR_OP_A2T1 && op <= CAGR_OP_A2T3))
                                ? 3 : 1;
-            k_piece_combine<<<(unsigned)t->num_split, 64, 0, st>>>(tp, t->split, no);
+            k_piece_combine<<<(unsigned)((t->num_split + kCombineWaves - 1) / kCombineWaves),
+                              64 * kCombineWaves, 0, st>>>(tp, t->split, t->num_split, no);
             CAGR_LAUNCHED();
         }
     } else if (q.mode == QM_EXPLICIT) {
