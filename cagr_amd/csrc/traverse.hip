@@ -1191,9 +1191,14 @@ __global__ void k_reset_counter(int32_t* c) {
 // depend on when the pieces ran), no values per query.  One wave per split
 // wave, four to a block; a wave's partials are loaded 8 slots at a time before
 // any is added (one memory round trip per 8 pieces, not one per piece).
+// NO is a template argument so that a query's values are written by one
+// store (the rows are scattered by the permutation: one store per row, not
+// one per value).
 constexpr int kCombineWaves = 4;
+template <int NO>
 __global__ __launch_bounds__(64 * kCombineWaves) void k_piece_combine(
-    TP p, const int4* __restrict__ split, int64_t nsplit, int no) {
+    TP p, const int4* __restrict__ split, int64_t nsplit) {
+    constexpr int no = NO;
     const int64_t ws = (int64_t)blockIdx.x * kCombineWaves + (threadIdx.x >> 6);
     if (ws >= nsplit) return;   // whole waves
     const int4 sw = split[ws];
@@ -1216,7 +1221,17 @@ __global__ __launch_bounds__(64 * kCombineWaves) void k_piece_combine(
             for (int k = 0; k < 3; ++k)
                 if (j0 + j < sw.z) acc[k] += v[j][k];
     }
-    for (int k = 0; k < no; ++k) store_out(p, no * oi + k, acc[k]);
+    if (p.odtype == CAGR_F32) {
+        struct Row { float v[NO]; };
+        Row r;
+        for (int k = 0; k < NO; ++k) r.v[k] = (float)acc[k];
+        ((Row*)p.out)[oi] = r;
+    } else {
+        struct Row { double v[NO]; };
+        Row r;
+        for (int k = 0; k < NO; ++k) r.v[k] = acc[k];
+        ((Row*)p.out)[oi] = r;
+    }
 }
 
 // Cost of the steps of each wave's self-query walk (the decisions of the FP64
@@ -1486,8 +1501,11 @@ int traverse(cagr_tree* t, int op, int precision, const Queries& q, int continuo
             const int no = (op == CAGR_OP_G || op == CAGR_OP_A2 || op == CAGR_OP_AT ||
                             op == CAGR_OP_A2T_SUM || (op >= CAGR_OP_A2T1 && op <= CAGR_OP_A2T3))
                                ? 3 : 1;
-            k_piece_combine<<<(unsigned)((t->num_split + kCombineWaves - 1) / kCombineWaves),
-                              64 * kCombineWaves, 0, st>>>(tp, t->split, t->num_split, no);
+            const unsigned cb = (unsigned)((t->num_split + kCombineWaves - 1) / kCombineWaves);
+            if (no == 3)
+                k_piece_combine<3><<<cb, 64 * kCombineWaves, 0, st>>>(tp, t->split, t->num_split);
+            else
+                k_piece_combine<1><<<cb, 64 * kCombineWaves, 0, st>>>(tp, t->split, t->num_split);
             CAGR_LAUNCHED();
         }
     } else if (q.mode == QM_EXPLICIT) {
