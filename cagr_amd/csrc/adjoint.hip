@@ -502,11 +502,21 @@ __global__ __launch_bounds__(kThreads) void k_far_gather(int64_t m, const int32_
     const int lane = threadIdx.x & 63;
     double z0 = 0.0, z1 = 0.0, z2 = 0.0;
     const int32_t hi = node_off[nd + 1];
-    for (int32_t i = node_off[nd] + lane; i < hi; i += 64) {
-        const float4 v = far_val[i];
-        z0 += (double)v.x;
-        z1 += (double)v.y;
-        z2 += (double)v.z;
+    // four of a lane's slots loaded before any is added (one memory round
+    // trip per 256 slots of the run), added in the lane's slot order
+    for (int32_t i0 = node_off[nd] + lane; i0 < hi; i0 += 256) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            v[u] = i0 + 64 * u < hi ? far_val[i0 + 64 * u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i0 + 64 * u < hi) {
+                z0 += (double)v[u].x;
+                z1 += (double)v[u].y;
+                z2 += (double)v[u].z;
+            }
+        }
     }
     wave_sum3(z0, z1, z2, z0, z1, z2);
     if (lane == 0) {
