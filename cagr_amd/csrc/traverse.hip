@@ -327,12 +327,17 @@ __device__ __forceinline__ void trav_v(
             n = __builtin_amdgcn_readfirstlane(n);
             const TopoRec t = rec32(topo, n);
             const NodeV32 nr = rec32(nodes, n);
+            // queries other than the tree's points: the FP64 centre (below)
+            double c[3] = {0.0, 0.0, 0.0};
+            if constexpr (QM != QM_SELF)
+                for (int k = 0; k < 3; ++k) c[k] = p.r64[3 * (int64_t)n + k];
             // the step's records loaded together and waited for once (the far
             // interaction's fields would otherwise be a second round trip)
             CAGR_LOADS_HERE("s"(t.escape), "s"(t.info), "s"(nr.cf[0]), "s"(nr.cf[1]), "s"(nr.cf[2]),
                             "s"(nr.hi2), "s"(nr.lo2), "s"(nr.m[0]), "s"(nr.m[1]), "s"(nr.m[2]),
                             "s"(nr.e[0]), "s"(nr.e[1]), "s"(nr.e[2]), "s"(nr.q[0]), "s"(nr.q[1]),
                             "s"(nr.q[2]));
+            if constexpr (QM != QM_SELF) CAGR_LOADS_HERE("s"(c[0]), "s"(c[1]), "s"(c[2]));
             __builtin_amdgcn_sched_barrier(0);
             // FP32 distance to the float centre: far above the record's hi2,
             // near below lo2; lanes between them take the FP64 test with the
@@ -341,9 +346,17 @@ __device__ __forceinline__ void trav_v(
             // the scalar unit (a ballot of a combined lane predicate, or a
             // predicate carried past the far branch, would go through a vector
             // register).
-            float d[3] = {xf[0] - nr.cf[0], xf[1] - nr.cf[1], xf[2] - nr.cf[2]};
-            if constexpr (QM != QM_SELF)
-                for (int k = 0; k < 3; ++k) d[k] += xl[k];
+            // d: the tree's own points are floats, their difference to the
+            // float centre is the FP32 one; other queries take the difference
+            // to the FP64 centre in FP64 (as many instructions as the float
+            // centre and the query's low part, without the centre's rounding,
+            // which weighs in the far terms of deep nodes)
+            float d[3];
+            if constexpr (QM == QM_SELF) {
+                for (int k = 0; k < 3; ++k) d[k] = xf[k] - nr.cf[k];
+            } else {
+                for (int k = 0; k < 3; ++k) d[k] = (float)(x[k] - c[k]);
+            }
             const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
             const uint64_t am = __builtin_amdgcn_ballot_w64(resume <= n);
             const uint64_t fm = __builtin_amdgcn_ballot_w64(r2 > nr.hi2);
@@ -713,12 +726,16 @@ __device__ __forceinline__ void trav_s(
                     for (int cc = 0; cc < 3; ++cc) fp[cc] += (cc == p.kdir) ? hg : 0.f;
                 }
             };
-            auto dist_set = [&](const NodeS32& nr, float* d) {
-                d[0] = xf[0] - nr.cf[0];
-                d[1] = xf[1] - nr.cf[1];
-                d[2] = xf[2] - nr.cf[2];
-                if constexpr (QM != QM_SELF)
-                    for (int cc = 0; cc < 3; ++cc) d[cc] += xl[cc];
+            // d to set k's centre: FP32 to the float centre for the tree's
+            // own points, FP64 to the FP64 centre for other queries (see
+            // trav_v)
+            auto dist_set = [&](int k, const NodeS32& nr, float* d) {
+                if constexpr (QM == QM_SELF) {
+                    for (int cc = 0; cc < 3; ++cc) d[cc] = xf[cc] - nr.cf[cc];
+                } else {
+                    const double* c64 = p.r64 + 3 * ((int64_t)k * p.nm + n);
+                    for (int cc = 0; cc < 3; ++cc) d[cc] = (float)(x[cc] - c64[cc]);
+                }
                 return fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
             };
             uint64_t amb[K], fmk[K];
@@ -730,7 +747,7 @@ __device__ __forceinline__ void trav_s(
                 // the FP64 test after the loop (open_band)
                 const NodeS32& nr = nrs[k];
                 float d[3];
-                const float r2 = dist_set(nr, d);
+                const float r2 = dist_set(k, nr, d);
                 // each comparison as its own wave mask, combined on the
                 // scalar unit (a ballot of a combined lane predicate would be
                 // taken through a vector register)
@@ -763,7 +780,7 @@ __device__ __forceinline__ void trav_s(
                             ++nfar;
                         } else {
                             float d[3];
-                            const float r2 = dist_set(nrs[k], d);
+                            const float r2 = dist_set(k, nrs[k], d);
                             far_set(k, nrs[k], d, r2);
                         }
                     }

@@ -111,7 +111,14 @@ class NormalOperator:
                                       precision=precision)
 
     def normal(self, p, precision):
-        return self.AT(self.A(p, precision), precision)
+        # the two negations of A and A^T cancel (exactly: IEEE arithmetic is
+        # odd-symmetric), so the normal operator applies neither
+        self.calls[precision] += 2
+        w = self.wt
+        y = w.apply("A2", p, self.w64, weights=self.ones64, an=self.an, precision=precision)
+        if self.plan and precision == "fp32c":
+            return w.apply_adjoint_planned("A2", y, self.w64, an=self.an, precision=precision)
+        return w.apply_adjoint("A2", y, self.w64, self.ones64, an=self.an, precision=precision)
 
 
 def _dot(a, b):
