@@ -193,14 +193,15 @@ class WindingNumberTreecode:
         return out
 
     def balance(self, op: str = "A2", attrs: Optional[torch.Tensor] = None, *,
-                piece_cost: float = 0.5, max_pieces: int = 8) -> np.ndarray:
-        """Dispatch the traversal's waves longest first (within each XCD's part
-        of the tree; the batched form's work items likewise), costed by the
-        exact interaction counts of `op` on `attrs` (default: unit
-        attributes), and cut the walks that cost more than `piece_cost` times
-        the mean into node-range pieces (at most `max_pieces` a wave; 0: no
-        pieces), so that the longest walks do not set the launch's length.
-        Returns the per-query cost used."""
+                piece_cost: float = 0.25, max_pieces: int = 16) -> np.ndarray:
+        """Balance the self-query walks: cut every walk that costs more than
+        `piece_cost` times the mean into node-range pieces of about that cost
+        (at most `max_pieces` a wave), dispatched in Morton order, so that
+        the longest walks do not set the launch's length; with
+        `piece_cost` = 0 (no pieces) the whole waves run longest first within
+        each XCD's part of the tree (the batched form's work items likewise),
+        costed by the exact interaction counts of `op` on `attrs` (default:
+        unit attributes).  Returns the per-query cost used."""
         if attrs is None:
             attrs = torch.ones((self.points.shape[0], IN_DIM[op]), dtype=self.points.dtype,
                                device=self.device)
