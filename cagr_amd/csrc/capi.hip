@@ -276,7 +276,9 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
     for (int64_t i = 0; i < t->num_items; ++i)
         for (int k = 0; k < items[i].w; ++k) cost[i] += query_cost[perm[items[i].z + k]];
     // kQueues contiguous chunks of the natural (Morton, cloud by cloud) order
-    // with equal total cost, each ordered longest first
+    // with equal total cost, each pulled in that order (neighbouring items
+    // running side by side share the caches; longest first within each chunk
+    // measured 1-3 % slower on 64 clouds)
     int64_t total = 0;
     for (int64_t i = 0; i < t->num_items; ++i) total += cost[i];
     int32_t bound[kQueues + 1];
@@ -288,14 +290,8 @@ int cagr_order_items(cagr_tree* t, const int64_t* query_cost) {
         bound[c] = (int32_t)i;
     }
     bound[kQueues] = (int32_t)t->num_items;
-    std::vector<int32_t> order(t->num_items);
-    for (int64_t j = 0; j < t->num_items; ++j) order[j] = (int32_t)j;
-    for (int c = 0; c < kQueues; ++c)
-        std::stable_sort(order.begin() + bound[c], order.begin() + bound[c + 1],
-                         [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
-    if (!t->item_order) CAGR_HIP(hipMalloc(&t->item_order, t->num_items * sizeof(int32_t)));
-    CAGR_HIP(hipMemcpy(t->item_order, order.data(), t->num_items * sizeof(int32_t),
-                       hipMemcpyHostToDevice));
+    (void)hipFree(t->item_order);
+    t->item_order = nullptr;
     for (int c = 0; c <= kQueues; ++c) t->item_bound[c] = bound[c];
     if (t->num_clouds == 1) {
         // the regular grid (one wave per block, block b on XCD b mod 8, the

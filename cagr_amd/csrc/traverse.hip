@@ -1123,9 +1123,8 @@ __global__ __launch_bounds__(kThreads, 8) void k_trav_s32(
 // queues until all are empty.  The pull order is cut into one chunk per XCD
 // (block b runs on XCD b mod 8): a chunk is a contiguous Morton range, so an
 // XCD's L2 holds the nodes and sources of its own part of space, and, once
-// the autotuner has costed the items, the chunks carry equal cost and are
-// ordered longest first.  A wave drains its own chunk, then steals from the
-// others.  Any block shape up to 512 threads.
+// the items are costed, the chunks carry equal cost.  A wave drains its own
+// chunk, then steals from the others.  Any block shape up to 512 threads.
 struct Items {
     const int4* items;
     const int32_t* order;        // pull position -> item (null: identity)

@@ -199,9 +199,10 @@ class WindingNumberTreecode:
         (at most `max_pieces` a wave), dispatched in Morton order, so that
         the longest walks do not set the launch's length; with
         `piece_cost` = 0 (no pieces) the whole waves run longest first within
-        each XCD's part of the tree (the batched form's work items likewise),
-        costed by the exact interaction counts of `op` on `attrs` (default:
-        unit attributes).  Returns the per-query cost used."""
+        each XCD's part of the tree; the batched form's work items get
+        per-XCD chunks of equal cost.  Costed by the exact interaction counts
+        of `op` on `attrs` (default: unit attributes).  Returns the per-query
+        cost used."""
         if attrs is None:
             attrs = torch.ones((self.points.shape[0], IN_DIM[op]), dtype=self.points.dtype,
                                device=self.device)

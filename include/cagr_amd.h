@@ -231,9 +231,12 @@ int cagr_forest_roots(const cagr_tree* tree, int64_t* node_offsets);
  * waves_per_block in [1, 8] and grid_blocks (0: 8 waves per SIMD on every
  * CU); persistent = 0 sets the regular grid's waves per block, in [1, 4]. */
 int cagr_set_launch(cagr_tree* tree, int persistent, int waves_per_block, int64_t grid_blocks);
-/* Longest-first order of the persistent work items from a per-query cost
- * (HOST int64 [n], original order, e.g. n_far + n_near from cagr_count);
- * NULL restores the natural order (and removes walk pieces). */
+/* Balances the work from a per-query cost (HOST int64 [n], original order,
+ * e.g. n_far + n_near from cagr_count): the persistent work items' per-XCD
+ * chunks (contiguous ranges of the Morton order) get equal cost, and the
+ * regular grid's whole waves run longest first within each XCD's share (when
+ * the walks are not cut into pieces); NULL restores the natural order and
+ * equal chunks (and removes walk pieces). */
 int cagr_order_items(cagr_tree* tree, const int64_t* query_cost);
 /* Walk pieces (no counterpart in the reference; a load-balancing setting of
  * this library): the self-query walk of a wave (64 consecutive sorted points)
@@ -244,7 +247,7 @@ int cagr_order_items(cagr_tree* tree, const int64_t* query_cost);
  * do not depend on their timing (they differ from the uncut walk's only by
  * the grouping of the sums).  Costed by the walk on the FP64 node centres of
  * the most recent upward pass (set 0: e.g. the one of cagr_count), pieces
- * dispatched longest first within each XCD's share.  Used by the
+ * dispatched in Morton order.  Used by the
  * regular-grid self-query traversals of single-cloud trees (one wave per
  * block); piece_cost <= 0 removes them.  counts (HOST int64 [2], may be
  * NULL) receives the number of pieces and of waves cut.  Synchronises the
