@@ -34,7 +34,7 @@ EXPORTS = (
     "cagr_apply_queries_noalloc", "cagr_apply_moments", "cagr_grid_query",
     "cagr_count", "cagr_forest_build", "cagr_forest_roots", "cagr_set_launch",
     "cagr_order_items", "cagr_split_walks", "cagr_profile", "cagr_profile_read",
-    "cagr_adjoint_plan",
+    "cagr_cg_step", "cagr_cg_work_len", "cagr_adjoint_plan",
     "cagr_apply_adjoint_planned", "cagr_pgr_build", "cagr_pgr_destroy", "cagr_pgr_info",
     "cagr_pgr_read", "cagr_pgr_box", "cagr_pgr_mesh", "cagr_pgr_mesh_read",
 )
@@ -64,6 +64,8 @@ _SIGS = {
     "cagr_split_walks": (_C, [_P, _D, _C, _P, _P]),
     "cagr_profile": (_C, [_P, _C]),
     "cagr_profile_read": (_C, [_P, _C, _P, _P, _P, _I]),
+    "cagr_cg_step": (_C, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
+    "cagr_cg_work_len": (_I, []),
     "cagr_adjoint_plan": (_C, [_P, _P, _C, _P]),
     "cagr_apply_adjoint_planned": (_C, [_P, _C, _C, _P, _C, _P, _C, _D, _P, _P]),
     "cagr_pgr_build": (_C, [_P, _I, _C, _C, _P, ct.POINTER(_P)]),

@@ -25,7 +25,8 @@ from typing import Optional
 import torch
 import torch.nn.functional as F
 
-from .wn_treecode import WindingNumberTreecode
+from ._lib import check
+from .wn_treecode import WindingNumberTreecode, _ptr, _stream
 
 PRESETS = {"l0": (0.002, 0.016), "l1": (0.01, 0.04), "l2": (0.02, 0.08), "l3": (0.03, 0.12),
            "l4": (0.04, 0.16), "l5": (0.05, 0.2)}
@@ -121,14 +122,30 @@ class NormalOperator:
         return w.apply_adjoint("A2", y, self.w64, self.ones64, an=self.an, precision=precision)
 
 
-def _dot(a, b):
-    return float((a * b).sum())
+class LibCGStep:
+    """One CG iteration's vector work by the library (``cagr_cg_step``) on
+    the vectors' device: step(x, r, p, q, rr, rr_new) with rr / rr_new 0-dim
+    FP64 device tensors (r.r before and after)."""
+
+    def __init__(self, lib, device):
+        self.L = lib
+        self.device = device
+        self.work = torch.empty(int(lib.cagr_cg_work_len()), dtype=torch.float64, device=device)
+
+    def __call__(self, x, r, p, q, rr, rr_new):
+        with torch.cuda.device(self.device):
+            check(self.L.cagr_cg_step(r.numel(), _ptr(x), _ptr(r), _ptr(p), _ptr(q),
+                                      rr.data_ptr(), rr_new.data_ptr(), _ptr(self.work),
+                                      self.work.numel(), _stream(self.device)))
 
 
 def _cg(op, rhs, x0, tol_abs, max_iter, precision, log=None, log_every=500, stall: int = 0,
-        check_every: int = 10):
+        check_every: int = 10, step=None):
     """CG on A^T A x = rhs (FP64 vectors); stops on the recurrence residual.
-    The scalars alpha and beta stay on the device and the host reads the
+    An iteration's vector work after q = A^T A p is ``step`` (default: the
+    library's ``cagr_cg_step`` for ``op.wt``, three launches, reductions in a
+    fixed order).  The
+    scalars alpha and beta stay on the device and the host reads the
     residual history only every ``check_every`` iterations, so the host never
     waits for the GPU inside an iteration (a run may take up to
     ``check_every`` - 1 iterations more than it needs; they only lower the
@@ -138,29 +155,26 @@ def _cg(op, rhs, x0, tol_abs, max_iter, precision, log=None, log_every=500, stal
     pair -- makes CG stagnate near its precision floor; the caller restarts
     from the true residual).  ``log(msg)`` (optional) gets the recurrence
     residual every ``log_every`` iterations."""
-    x = x0.clone()
-    r = rhs - op.normal(x, precision) if x0.abs().max() > 0 else rhs.clone()
+    x = x0.clone().contiguous()
+    r = (rhs - op.normal(x, precision) if x0.abs().max() > 0 else rhs.clone()).contiguous()
     p = r.clone()
-    rv, pv = r.view(-1), p.view(-1)
-    rr = torch.dot(rv, rv)
-    hist = torch.empty(check_every, dtype=rr.dtype, device=rr.device)
+    rv = r.view(-1)
+    # hist[0] = r.r before the block, hist[j + 1] = r.r after its iteration j
+    hist = torch.empty(check_every + 1, dtype=torch.float64, device=r.device)
+    hist[0] = torch.dot(rv, rv)
+    if step is None:
+        step = LibCGStep(op.wt.L, r.device)
     it = 0
-    best, since = float(rr), 0
+    best, since = float(hist[0]), 0
     if best ** 0.5 <= tol_abs:
         return x, 0
     while it < max_iter:
         n = min(check_every, max_iter - it)
         for j in range(n):
-            q = op.normal(p, precision)
-            pq = torch.dot(pv, q.view(-1))
-            alpha = torch.where(pq != 0, rr / pq, torch.zeros_like(pq))
-            x.addcmul_(p, alpha)
-            r.addcmul_(q, alpha, value=-1.0)
-            rr_new = torch.dot(rv, rv)
-            p.mul_(rr_new / rr).add_(r)
-            rr = rr_new
-            hist[j] = rr
-        rrs = hist[:n].tolist()     # the one host read of the block
+            q = op.normal(p, precision).contiguous()
+            step(x, r, p, q, hist[j], hist[j + 1])
+        rrs = hist[1:n + 1].tolist()   # the one host read of the block
+        hist[0] = hist[n]
         done = False
         for j, v in enumerate(rrs):
             k = it + j + 1

@@ -266,6 +266,20 @@ int cagr_profile(cagr_tree* tree, int enable);
 int cagr_profile_read(cagr_tree* tree, int kind, int64_t* launches, double* total_ms,
                       double* each_ms, int64_t cap);
 
+/* One iteration's vector work of a conjugate-gradient loop in FP64 (no
+ * counterpart in the reference's operator: its solve writes these updates as
+ * torch tensor expressions, main_CAGR.py:94-144), on DEVICE vectors of len
+ * doubles, after the caller has applied q = A^T A p:
+ *   pq = p.q, alpha = *rr / pq (0 if pq = 0), x += alpha p, r -= alpha q,
+ *   *rr_new = r.r, p = r + (*rr_new / *rr) p.
+ * rr and rr_new: distinct DEVICE scalars; work: DEVICE, cagr_cg_work_len()
+ * doubles.  Every reduction runs in a fixed order (the iterates do not depend
+ * on scheduling); three launches on the stream, no synchronisation. */
+int cagr_cg_step(int64_t len, double* x, double* r, double* p, const double* q,
+                 const double* rr, double* rr_new, double* work, int64_t work_len,
+                 void* stream);
+int64_t cagr_cg_work_len(void);
+
 /* GaussRecon's PGR adaptive octree over oriented samples and its query
  * corners: replaces Octree::setTree(file, d, d_min) and the gridDataVector it
  * leaves (gaussrecon_src/Octree.cpp:77-400, used at

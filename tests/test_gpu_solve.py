@@ -10,6 +10,8 @@
   through the adjoint plan, FP64 refinement and stopping test) both reach
   ||A^T(b - A mu)|| / ||A^T b|| <= 1e-3, checked again with the FP64
   operator.
+* CG step: the library's fused vector work of one CG iteration against the
+  same recurrence in torch FP64.
 """
 import numpy as np
 import pytest
@@ -194,3 +196,45 @@ def test_cgnr_fp64_and_mixed_reach_tolerance():
     fit64 = float(torch.linalg.norm(b - op.A(mu64, "fp64")))
     fit32 = float(torch.linalg.norm(b - op.A(mu32, "fp64")))
     assert fit32 <= 1.05 * fit64
+
+
+@pytest.mark.parametrize("length", [3, 1000, 3_000_000])
+def test_cg_step_matches_the_recurrence(length):
+    """cagr_cg_step (the solves' CG vector work, reductions in a fixed order)
+    against the same recurrence in torch FP64 on random vectors, lengths from
+    below one block to the 1M-point solve's 3M values: within 1e-13
+    relative (only the summation order of the two dot products differs), and
+    the same bits when repeated (deterministic); pq = 0 gives alpha = 0."""
+    from cagr_amd.solve import LibCGStep
+    say("cg step", length)
+    g = torch.Generator(device="cpu").manual_seed(length)
+    x, r, p, e = (torch.randn(length, dtype=torch.float64, generator=g).to(DEV) for _ in range(4))
+    q = 2.0 * p + 0.1 * e     # p.q well away from 0, as for an SPD operator
+    rr = torch.dot(r, r)
+    step = LibCGStep(WindingNumberTreecode(cloud(500, 3)).L, torch.device(DEV))
+
+    def run(fn):
+        xs, rs, ps = x.clone(), r.clone(), p.clone()
+        hist = torch.empty(2, dtype=torch.float64, device=DEV)
+        hist[0] = rr
+        fn(xs, rs, ps, q, hist[0], hist[1])
+        torch.cuda.synchronize()
+        return xs, rs, ps, hist[1].clone()
+
+    def ref(xs, rs, ps, qv, rr0, rr1):
+        pq = torch.dot(ps, qv)
+        alpha = rr0 / pq
+        xs.add_(alpha * ps)
+        rs.sub_(alpha * qv)
+        rr1.copy_(torch.dot(rs, rs))
+        ps.mul_(rr1 / rr0).add_(rs)
+
+    got, want, again = run(step), run(ref), run(step)
+    for a, b, c in zip(got, want, again):
+        assert torch.equal(a, c)
+        assert float((a - b).abs().max()) <= 1e-13 * max(float(b.abs().max()), 1e-300)
+    # p orthogonal to q: alpha = 0, x and r unchanged
+    qz = torch.zeros_like(q)
+    xs, rs, ps, rn = (lambda: run(lambda a, b, c, d, e, f: step(a, b, c, qz, e, f)))()
+    assert torch.equal(xs, x) and torch.equal(rs, r)
+    assert abs(float(rn) - float(rr)) <= 1e-13 * float(rr)

@@ -4,7 +4,22 @@ operator: convergence to the requested recurrence residual, the iteration
 bound, the periodic residual checks and the stall rule."""
 import torch
 
-from cagr_amd.solve import _cg
+from cagr_amd.solve import _cg as _cg_lib
+
+
+def torch_step(x, r, p, q, rr, rr_new):
+    """The CG recurrence of cagr_cg_step in torch (the library's step runs on
+    a GPU; these tests drive the loop's host logic with it on the CPU)."""
+    pq = torch.dot(p.view(-1), q.view(-1))
+    alpha = rr / pq if float(pq) != 0.0 else torch.zeros_like(pq)
+    x.add_(alpha * p)
+    r.sub_(alpha * q)
+    rr_new.copy_(torch.dot(r.view(-1), r.view(-1)))
+    p.mul_(rr_new / rr if float(rr) != 0.0 else 0.0).add_(r)
+
+
+def _cg(*a, **k):
+    return _cg_lib(*a, step=torch_step, **k)
 
 
 class DenseNormal:
