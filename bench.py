@@ -11,8 +11,10 @@ One step = one matvec pair: A.mu (A21, A22, A23 on mu [N,3], one fused
 upward pass + traversal) followed by A^T.r (A2T1, A2T2, A2T3 on the three
 columns of r = -A.mu, summed; one fused pass with three sets of centres), with
 the inputs resident in HBM.  Upward passes are inside the step; the tree
-build and the ordering of the traversal waves (longest first, from exact
-interaction counts) are outside (build_ms, balance_ms).  The main figure is the
+build and the balancing of the walks (cut into pieces costed from exact
+interaction counts) are outside (build_ms, balance_ms).  The timed steps run
+without instrumentation; the per-launch HIP events of the roofline come from
+the same steps run once more.  The main figure is the
 FP32-compensated form; the plain FP64 operator on the same workload is timed
 beside it (fp64_value), and the oracle (the FP64 CPU restatement) is timed on
 a bounded sample of queries on the host cores (cpu_baseline).
@@ -168,13 +170,14 @@ def main():
     build_ms = (time.perf_counter() - t0) * 1e3
     info = wt.info()
     log(f"tree built in {build_ms:.1f} ms: {info}")
-    # dispatch order of the traversal waves, longest first, costed by the exact
-    # interaction counts of A2 on mu (setup, like the build; outside the timed steps)
+    # the walks cut into pieces of about a quarter of the mean wave cost
+    # (WindingNumberTreecode.balance, costed on A2 with mu; setup, like the
+    # build; outside the timed steps)
     t0 = time.perf_counter()
     wt.balance("A2", mu)
     torch.cuda.synchronize()
     balance_ms = (time.perf_counter() - t0) * 1e3
-    log(f"waves ordered longest first in {balance_ms:.1f} ms")
+    log(f"walks balanced in {balance_ms:.1f} ms")
 
     def step(prec):
         a = wt.forward_A2(mu, w, precision=prec)
@@ -199,7 +202,10 @@ def main():
 
     # ---------------------------------------------------------------- FP32c
     log(f"fp32c: {args.warmup} warmup + {args.steps} timed steps")
-    el, (r32, a32), prof = timed("fp32c", args.steps, args.warmup, profile=True)
+    # the timed steps carry no instrumentation; the per-launch HIP events of
+    # the roofline come from the same number of steps run once more
+    el, (r32, a32), _ = timed("fp32c", args.steps, args.warmup)
+    _, _, prof = timed("fp32c", args.steps, 0, profile=True)
     value = args.steps / el
     ms_per_step = el / args.steps * 1e3
     trav = np.array(prof["traverse"]).reshape(args.steps, 2)   # [A2, A2T_sum] per step
@@ -228,7 +234,8 @@ def main():
     if not args.no_fp64:
         ks = args.fp64_steps or max(2, args.steps // 4)
         log(f"fp64: 1 warmup + {ks} timed steps")
-        el64, (r64, a64), prof64 = timed("fp64", ks, 1, profile=True)
+        el64, (r64, a64), _ = timed("fp64", ks, 1)
+        _, _, prof64 = timed("fp64", ks, 0, profile=True)
         tr64 = np.array(prof64["traverse"]).reshape(ks, 2)
         # same inputs for both precisions (A^T.r is applied to the FP32c A.mu:
         # its node centres follow |r|, so different inputs give different trees)
@@ -410,8 +417,8 @@ def main():
 
 def seed_run(args, seed, with_fp64, with_solve):
     """The 1M workload on another seed: same setup as the main run (tree
-    build, waves ordered longest first), then timed FP32c and FP64 pairs and
-    the two tolerance solves."""
+    build, walks balanced), then timed FP32c and FP64 pairs and the two
+    tolerance solves."""
     import torch
     from cagr_amd import WindingNumberTreecode, synth
     from cagr_amd.solve import cgnr_solve, cgnr_solve_mixed
