@@ -15,11 +15,12 @@
 // (find_next + nonleaf per level, cuda_iface:77-131), this runs
 // 1 + ceil(depth/3) launches: one over the leaves (points staged through LDS,
 // the points' per-call source records written on the way), then one per band
-// of up to 3 depths, deepest first, a wave per node at the band's top depth
+// of up to 3 depths, deepest first, a block per node at the band's top depth
 // taking the band's internal nodes level by level.
 //
 // Several "sets" (K) of scalar attributes can share one pass (the three
-// inputs of the fused sum_k A2Tk): each set has its own weights and centres.
+// inputs of the fused sum_k A2Tk): each set has its own weights and centres,
+// and its own wave in every block of both launches.
 #include "common.h"
 
 namespace cagr {
@@ -188,12 +189,12 @@ __device__ inline void load_point(PtIn<KC, P>& q, int32_t s, int64_t j,
     q.w = weights ? wload(weights, wdtype, j) : 0.0;
 }
 
-// a leaf's moments from its sums over its points: centre = weighted mean,
-// or the plain mean when the total weight is 0
+// a leaf's moments of set k from its sums over its points: centre =
+// weighted mean, or the plain mean when the total weight is 0
 template <int FAM, int K, int P>
-__device__ inline void leaf_finish(int32_t nd, const TopoRec& t, const double (&tw)[K],
-                                   const double (&rep)[K][3],
-                                   const double (&acc)[K][Rec<FAM, K, P>::C],
+__device__ inline void leaf_finish(int32_t nd, const TopoRec& t, int k, double tw,
+                                   const double (&rep)[3],
+                                   const double (&acc)[Rec<FAM, K, P>::C],
                                    const double (&rep0)[3], int64_t m, double* __restrict__ w64,
                                    double* __restrict__ r64, double* __restrict__ m64,
                                    void* __restrict__ nodes, double an, int dir0,
@@ -201,21 +202,19 @@ __device__ inline void leaf_finish(int32_t nd, const TopoRec& t, const double (&
 #pragma clang fp contract(off)
     constexpr int C = Rec<FAM, K, P>::C;
     const double np = (double)(t.end - t.begin);
-    for (int k = 0; k < K; ++k) {
-        double c[3];
-        if (tw[k] > 0) {
-            const double inv = 1.0 / tw[k];
-            for (int d = 0; d < 3; ++d) c[d] = rep[k][d] * inv;
-        } else {
-            const double inv = 1.0 / np;
-            for (int d = 0; d < 3; ++d) c[d] = rep0[d] * inv;
-        }
-        const int64_t o = (int64_t)k * m + nd;
-        w64[o] = tw[k];
-        for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
-        for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[k][cc];
-        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2, qr);
+    double c[3];
+    if (tw > 0) {
+        const double inv = 1.0 / tw;
+        for (int d = 0; d < 3; ++d) c[d] = rep[d] * inv;
+    } else {
+        const double inv = 1.0 / np;
+        for (int d = 0; d < 3; ++d) c[d] = rep0[d] * inv;
     }
+    const int64_t o = (int64_t)k * m + nd;
+    w64[o] = tw;
+    for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
+    for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[cc];
+    Rec<FAM, K, P>::node(nodes, nd, k, c, acc, an, dir0, t.thr2, qr);
 }
 
 // child ids of internal node t in octant order (ascending id): positions
@@ -227,65 +226,67 @@ __device__ inline void load_kids(const TopoRec& t, const int32_t* __restrict__ o
     for (int j = 0; j < 8; ++j) ch[j] = order[t.kids + (j < nk ? j : nk - 1)];
 }
 
-// internal node nd from its children ch[0 .. nk): every set's child moments
+// set k of internal node nd from its children ch[0 .. nk): the child moments
 // are loaded and summed in order before any store
 template <int FAM, int K, int P>
-__device__ inline void up_internal_node(int32_t nd, const TopoRec& t, const int32_t ch[8],
+__device__ inline void up_internal_node(int32_t nd, const TopoRec& t, const int32_t ch[8], int k,
                                         int64_t m, double* __restrict__ w64,
                                         double* __restrict__ r64, double* __restrict__ m64,
                                         void* __restrict__ nodes, double an, int dir0,
-                                   double qr) {
+                                        double qr) {
 #pragma clang fp contract(off)
     constexpr int C = Rec<FAM, K, P>::C;
     const int nk = topo_nkids(t);
-    double tw[K], rep[K][3], rep0[K][3], acc[K][C];
-    for (int k = 0; k < K; ++k) {
-        tw[k] = 0.0;
-        for (int d = 0; d < 3; ++d) { rep[k][d] = 0.0; rep0[k][d] = 0.0; }
-        for (int c = 0; c < C; ++c) acc[k][c] = 0.0;
+    double tw = 0.0, rep[3] = {0.0, 0.0, 0.0}, rep0[3] = {0.0, 0.0, 0.0}, acc[C];
+    for (int c = 0; c < C; ++c) acc[c] = 0.0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int64_t o = (int64_t)k * m + ch[j];
-            const double wc = w64[o];
-            double mc[C], cd[3];
-            for (int c = 0; c < C; ++c) mc[c] = m64[C * o + c];
-            for (int d = 0; d < 3; ++d) cd[d] = r64[3 * o + d];
-            if (j < nk) {
-                tw[k] += wc;
-                for (int c = 0; c < C; ++c) acc[k][c] += mc[c];
-                for (int d = 0; d < 3; ++d) {
-                    rep[k][d] += cd[d] * wc;
-                    rep0[k][d] += cd[d];
-                }
+    for (int j = 0; j < 8; ++j) {
+        const int64_t o = (int64_t)k * m + ch[j];
+        const double wc = w64[o];
+        double mc[C], cd[3];
+        for (int c = 0; c < C; ++c) mc[c] = m64[C * o + c];
+        for (int d = 0; d < 3; ++d) cd[d] = r64[3 * o + d];
+        if (j < nk) {
+            tw += wc;
+            for (int c = 0; c < C; ++c) acc[c] += mc[c];
+            for (int d = 0; d < 3; ++d) {
+                rep[d] += cd[d] * wc;
+                rep0[d] += cd[d];
             }
         }
     }
     const double np = (double)(t.end - t.begin);
-    for (int k = 0; k < K; ++k) {
-        double c[3];
-        if (tw[k] > 0) {
-            const double inv = 1.0 / tw[k];
-            for (int d = 0; d < 3; ++d) c[d] = rep[k][d] * inv;
-        } else {
-            const double inv = 1.0 / np;
-            for (int d = 0; d < 3; ++d) c[d] = rep0[k][d] * inv;
-        }
-        const int64_t o = (int64_t)k * m + nd;
-        w64[o] = tw[k];
-        for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
-        for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[k][cc];
-        Rec<FAM, K, P>::node(nodes, nd, k, c, acc[k], an, dir0, t.thr2, qr);
+    double c[3];
+    if (tw > 0) {
+        const double inv = 1.0 / tw;
+        for (int d = 0; d < 3; ++d) c[d] = rep[d] * inv;
+    } else {
+        const double inv = 1.0 / np;
+        for (int d = 0; d < 3; ++d) c[d] = rep0[d] * inv;
     }
+    const int64_t o = (int64_t)k * m + nd;
+    w64[o] = tw;
+    for (int d = 0; d < 3; ++d) r64[3 * o + d] = c[d];
+    for (int cc = 0; cc < C; ++cc) m64[C * o + cc] = acc[cc];
+    Rec<FAM, K, P>::node(nodes, nd, k, c, acc, an, dir0, t.thr2, qr);
 }
 
-// Leaf launch, one wave per 64 consecutive node ids (preorder).  The leaves
-// among them cover one run of sorted points with nothing between them (leaves
-// in preorder hold consecutive point ranges), so the run is staged through
-// LDS T points at a time with coalesced loads (the attributes are gathered by
-// the permutation), the points' source records written on the way; then each
-// leaf's lane sums its points from LDS in order.  Every point is staged once.
+// Leaf launch: a block of K waves per kLeafIds consecutive node ids
+// (preorder), wave k summing set k.  The leaves among the ids cover one run
+// of sorted points with nothing between them (leaves in preorder hold
+// consecutive point ranges), so the run is staged through LDS T points at a
+// time with coalesced loads by the whole block (the attributes are gathered
+// by the permutation), the points' source records written on the way; then
+// each leaf's lane in wave k sums set k over its points from LDS in order.
+// Every point is staged once.  The sums are serial per leaf and set, so a
+// wave per set shortens each block's sum phase K-fold.  (16 ids per block
+// instead of 64 measured the same: the launch is held by its HBM traffic,
+// about twice the compulsory bytes, the attributes' gather by the
+// permutation fetching a line per point.)
+constexpr int kLeafIds = 64;
+
 template <typename TI, int FAM, int K, int P>
-__global__ __launch_bounds__(64, 2) void k_up_leaves(
+__global__ __launch_bounds__(64 * K) void k_up_leaves(
     int64_t m, const TopoRec* __restrict__ topo, const int32_t* __restrict__ perm,
     const double* __restrict__ pts64, const float* __restrict__ pts32,
     const TI* __restrict__ attrs, const void* __restrict__ weights, int wdtype,
@@ -296,54 +297,57 @@ __global__ __launch_bounds__(64, 2) void k_up_leaves(
     constexpr int KC = K * C;
     constexpr int F = 3 + KC + K;  // staged per point: y, attributes, a weight per set
     constexpr int T = 256;         // points per staged tile
-    constexpr int U = T / 64;
+    constexpr int NT = 64 * K;     // threads of the block
+    constexpr int U = (T + NT - 1) / NT;
     __shared__ double s_pt[F * T];
-    const int32_t lane = (int32_t)threadIdx.x;
-    const int64_t nd = (int64_t)blockIdx.x * 64 + lane;
+    const int32_t tid = (int32_t)threadIdx.x;
+    const int32_t lane = tid & 63;
+    const int k = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's set
+    const int64_t nd = (int64_t)blockIdx.x * kLeafIds + lane;
     TopoRec t{};
     bool leaf = false;
-    if (nd < m) {
+    if (lane < kLeafIds && nd < m) {
         t = topo[nd];
         leaf = topo_leaf(t);
     }
-    const uint64_t lm = __ballot(leaf);
+    const uint64_t lm = __ballot(leaf);   // the same in every wave of the block
     if (lm == 0) return;
     const int first = __ffsll((unsigned long long)lm) - 1;
     const int last = 63 - __clzll((unsigned long long)lm);
     const int32_t lo = __shfl(t.begin, first), hi = __shfl(t.end, last);
-    double tw[K], rep[K][3], acc[K][C], rep0[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < K; ++k) {
-        tw[k] = 0.0;
-        for (int d = 0; d < 3; ++d) rep[k][d] = 0.0;
-        for (int c = 0; c < C; ++c) acc[k][c] = 0.0;
-    }
+    double tw = 0.0, rep[3] = {0.0, 0.0, 0.0}, acc[C], rep0[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < C; ++c) acc[c] = 0.0;
     for (int32_t tl = lo; tl < hi; tl += T) {
         const int32_t th = min(tl + T, hi);
         PtIn<KC, P> pt[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t s = min(tl + lane + 64 * u, th - 1);
-            load_point<TI, KC, P>(pt[u], s, (int64_t)perm[s], pts64, pts32, attrs, weights, wdtype);
+            const int32_t i = tid + NT * u;
+            if (i < T) {
+                const int32_t s = min(tl + i, th - 1);
+                load_point<TI, KC, P>(pt[u], s, (int64_t)perm[s], pts64, pts32, attrs, weights,
+                                      wdtype);
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int32_t s = tl + lane + 64 * u;
-            if (s < th) {
+            const int32_t i = tid + NT * u;
+            const int32_t s = tl + i;
+            if (i < T && s < th) {
                 Rec<FAM, K, P>::src(src, s, pt[u].y, pt[u].y32, pt[u].a, an, dir0);
-                const int i = s - tl;
                 for (int d = 0; d < 3; ++d) s_pt[d * T + i] = pt[u].y[d];
                 for (int c = 0; c < KC; ++c) s_pt[(3 + c) * T + i] = pt[u].a[c];
-                for (int k = 0; k < K; ++k) {
+                for (int kk = 0; kk < K; ++kk) {
                     double w;
                     if (weights) {
                         w = pt[u].w;
                     } else {
-                        const double* a = pt[u].a + k * C;
+                        const double* a = pt[u].a + kk * C;
                         double ss = a[0] * a[0];
                         for (int c = 1; c < C; ++c) ss = ss + a[c] * a[c];
                         w = sqrt(ss);
                     }
-                    s_pt[(3 + KC + k) * T + i] = w;
+                    s_pt[(3 + KC + kk) * T + i] = w;
                 }
             }
         }
@@ -353,36 +357,37 @@ __global__ __launch_bounds__(64, 2) void k_up_leaves(
             for (int32_t s = max(t.begin, tl); s < a1; ++s) {
                 const int i = s - tl;
                 const double y[3] = {s_pt[i], s_pt[T + i], s_pt[2 * T + i]};
-                for (int k = 0; k < K; ++k) {
-                    const double w = s_pt[(3 + KC + k) * T + i];
-                    tw[k] += w;
-                    for (int c = 0; c < C; ++c) acc[k][c] += s_pt[(3 + k * C + c) * T + i];
-                    for (int d = 0; d < 3; ++d) rep[k][d] += y[d] * w;
-                }
+                const double w = s_pt[(3 + KC + k) * T + i];
+                tw += w;
+                for (int c = 0; c < C; ++c) acc[c] += s_pt[(3 + k * C + c) * T + i];
+                for (int d = 0; d < 3; ++d) rep[d] += y[d] * w;
                 for (int d = 0; d < 3; ++d) rep0[d] += y[d];
             }
         }
         __syncthreads();
     }
     if (leaf)
-        leaf_finish<FAM, K, P>((int32_t)nd, t, tw, rep, acc, rep0, m, w64, r64, m64, nodes, an,
+        leaf_finish<FAM, K, P>((int32_t)nd, t, k, tw, rep, acc, rep0, m, w64, r64, m64, nodes, an,
                                dir0, qr);
 }
 
-// Band launch, one wave per node at the band's top depth: the internal nodes
-// of the band's levels nlev-1..0 (nlev <= 3) under it, with a barrier between
-// levels; the leaves were done by the leaf launch.  Level l holds at most 8^l
-// <= 64 of them, so each lane has at most one node per level, and the node ids
-// and child ids of the next level (which no launch here writes) are loaded
-// while the current level runs.
+// Band launch, a block of K waves per node at the band's top depth (wave k
+// for set k): the internal nodes of the band's levels nlev-1..0 (nlev <= 3)
+// under it, with a barrier between levels; the leaves were done by the leaf
+// launch.  Level l holds at most 8^l <= 64 of them, so each lane has at most
+// one node per level, and the node ids and child ids of the next level (which
+// no launch here writes) are loaded while the current level runs.
 template <int FAM, int K, int P>
-__global__ __launch_bounds__(64) void k_up_band(const int2* __restrict__ seg, int nlev,
-                                                const int32_t* __restrict__ order,
-                                                const TopoRec* __restrict__ topo, int64_t m,
-                                                double* __restrict__ w64, double* __restrict__ r64,
-                                                double* __restrict__ m64, void* __restrict__ nodes,
-                                                double an, int dir0, double qr) {
-    const int32_t lane = (int32_t)threadIdx.x;
+__global__ __launch_bounds__(64 * K) void k_up_band(const int2* __restrict__ seg, int nlev,
+                                                    const int32_t* __restrict__ order,
+                                                    const TopoRec* __restrict__ topo, int64_t m,
+                                                    double* __restrict__ w64,
+                                                    double* __restrict__ r64,
+                                                    double* __restrict__ m64,
+                                                    void* __restrict__ nodes, double an, int dir0,
+                                                    double qr) {
+    const int32_t lane = (int32_t)(threadIdx.x & 63);
+    const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // this wave's set
     auto fetch = [&](int l, int32_t& nd, TopoRec& t, int32_t ch[8]) {
         const int2 sg = seg[(int64_t)blockIdx.x * nlev + l];
         nd = -1;
@@ -402,7 +407,8 @@ __global__ __launch_bounds__(64) void k_up_band(const int2* __restrict__ seg, in
         int32_t nd2 = -1, ch2[8];
         TopoRec t2;
         if (l > 0) fetch(l - 1, nd2, t2, ch2);
-        if (nd >= 0) up_internal_node<FAM, K, P>(nd, t, ch, m, w64, r64, m64, nodes, an, dir0, qr);
+        if (nd >= 0)
+            up_internal_node<FAM, K, P>(nd, t, ch, k, m, w64, r64, m64, nodes, an, dir0, qr);
         __syncthreads();
         nd = nd2;
         t = t2;
@@ -418,12 +424,12 @@ int run_upward(cagr_tree* t, const TI* attrs, const void* weights, int wdtype, d
     bool ok = true;
     for (const int4& b : t->up_bands) ok = ok && b.y >= 1 && b.y <= 3;  // <= 64 nodes a level
     CAGR_REQUIRE(ok, CAGR_ERR_STATE, "upward: bad band");
-    k_up_leaves<TI, FAM, K, P><<<(unsigned)((m + 63) / 64), 64, 0, st>>>(
+    k_up_leaves<TI, FAM, K, P><<<(unsigned)((m + kLeafIds - 1) / kLeafIds), 64 * K, 0, st>>>(
         m, t->topo, t->perm, t->pts64, t->pts32, attrs, weights, wdtype, t->w64, t->r64, t->m64,
         t->nodes, t->src, an, dir0, t->qround);
     CAGR_LAUNCHED();
     for (const int4& b : t->up_bands) {
-        k_up_band<FAM, K, P><<<b.z, 64, 0, st>>>(t->up_seg + b.w, b.y, t->order, t->topo, m, t->w64,
+        k_up_band<FAM, K, P><<<b.z, 64 * K, 0, st>>>(t->up_seg + b.w, b.y, t->order, t->topo, m, t->w64,
                                                  t->r64, t->m64, t->nodes, an, dir0, t->qround);
         CAGR_LAUNCHED();
     }
