@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 namespace cagr {
 namespace {
@@ -493,31 +494,41 @@ __global__ __launch_bounds__(kThreads) void k_adj32_list(AP p, const TopoRec* __
 
 // L of every node from its run of slots, one wave per node: lanes stride the
 // run (coalesced), then a fixed-order wave reduction in FP64, so the planned
-// adjoint is deterministic
-__global__ __launch_bounds__(kThreads) void k_far_gather(int64_t m, const int32_t* __restrict__ node_off,
+// adjoint is deterministic.  Waves take the nodes in depth order (order[],
+// shallowest first): the top nodes, whose runs hold a slot for nearly every
+// wave of the walk, start first instead of wherever their preorder ids fall.
+__global__ __launch_bounds__(kThreads) void k_far_gather(int64_t m, const int32_t* __restrict__ order,
+                                                         const int32_t* __restrict__ node_off,
                                                          const float4* __restrict__ far_val,
                                                          double* __restrict__ adj_node) {
-    const int64_t nd = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (nd >= m) return;   // whole waves
+    const int64_t k = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (k >= m) return;   // whole waves
+    const int64_t nd = order[k];
     const int lane = threadIdx.x & 63;
     double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-    const int32_t hi = node_off[nd + 1];
-    // four of a lane's slots loaded before any is added (one memory round
-    // trip per 256 slots of the run), added in the lane's slot order
-    for (int32_t i0 = node_off[nd] + lane; i0 < hi; i0 += 256) {
-        float4 v[4];
+    const int32_t lo = node_off[nd], hi = node_off[nd + 1];
+    // B of a lane's slots loaded before any is added (one memory round trip
+    // per 64 B slots of the run), added in the lane's slot order whatever B:
+    // 16 for long runs (the top nodes have a slot for nearly every wave), 4
+    // for the rest
+    auto batch = [&](int32_t i0, auto bc) {
+        constexpr int B = decltype(bc)::value;
+        float4 v[B];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < B; ++u)
             v[u] = i0 + 64 * u < hi ? far_val[i0 + 64 * u] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < B; ++u) {
             if (i0 + 64 * u < hi) {
                 z0 += (double)v[u].x;
                 z1 += (double)v[u].y;
                 z2 += (double)v[u].z;
             }
         }
-    }
+    };
+    int32_t i0 = lo + lane;
+    for (; i0 + 64 * 15 < hi; i0 += 64 * 16) batch(i0, std::integral_constant<int, 16>{});
+    for (; i0 < hi; i0 += 256) batch(i0, std::integral_constant<int, 4>{});
     wave_sum3(z0, z1, z2, z0, z1, z2);
     if (lane == 0) {
         adj_node[3 * nd] = z0;
@@ -778,7 +789,7 @@ int adjoint(cagr_tree* t, int op, int precision, const void* g, int dtype, const
     CAGR_LAUNCHED();
     if (slots) {
         k_far_gather<<<(m + kWavesPerBlock - 1) / kWavesPerBlock, kThreads, 0, st>>>(
-            m, t->plan_node_off, t->plan_far_val, t->adj_node);
+            m, t->order, t->plan_node_off, t->plan_far_val, t->adj_node);
         CAGR_LAUNCHED();
     }
     const int64_t nb = (n + kThreads - 1) / kThreads;
