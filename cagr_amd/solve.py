@@ -197,13 +197,21 @@ def _cg(op, rhs, x0, tol_abs, max_iter, precision, log=None, log_every=500, stal
 
 
 def _cgnr(wt: WindingNumberTreecode, width: float, tol: float, b: float, max_iter: int,
-          max_restarts: int, an: float, inner: str, plan: bool, log, stall: int):
+          max_restarts: int, an: float, inner: str, plan: bool, log, stall: int,
+          tree_order: bool = True):
     """CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol, verified with the FP64
     operator: CG on the correction equation with the ``inner`` operator until
     it converges or stalls (``stall`` iterations without a 10 % gain; 0 =
-    never), then the true residual in FP64 and a restart from it."""
+    never), then the true residual in FP64 and a restart from it.  With
+    ``tree_order`` the solve runs on the cloud in its tree's point order
+    (``in_tree_order``, built inside the timed region: the same tree, every
+    operator's vectors read and written without the permutation) and the
+    solution is returned in the input order."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    perm = None
+    if tree_order:
+        wt, perm = wt.in_tree_order()
     op = NormalOperator(wt, width, an, plan=plan)
     n = wt.points.shape[0]
     bt = torch.full((n, 3), b, dtype=torch.float64, device=wt.device)
@@ -223,6 +231,10 @@ def _cgnr(wt: WindingNumberTreecode, width: float, tol: float, b: float, max_ite
         mu += d
         r = op.AT(bt - op.A(mu, "fp64"), "fp64")
         restarts += 1
+    if perm is not None:
+        out = torch.empty_like(mu)
+        out[perm] = mu
+        mu = out
     torch.cuda.synchronize()
     return mu, {"iters": iters, "restarts": restarts - 1, "rel_residual": res, "history": history,
                 "seconds": time.perf_counter() - t0, "calls": dict(op.calls)}
@@ -230,15 +242,18 @@ def _cgnr(wt: WindingNumberTreecode, width: float, tol: float, b: float, max_ite
 
 def cgnr_solve(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
                max_iter: int = 5000, an: float = 1.0, log=None, stall: int = 1500,
-               max_restarts: int = 10):
+               max_restarts: int = 10, tree_order: bool = True):
     """Plain FP64 CGNR to ||A^T(b - A mu)|| / ||A^T b|| <= tol, with the same
-    restart rule as the mixed solve (``stall`` = 0: one CG run)."""
-    return _cgnr(wt, width, tol, b, max_iter, max_restarts, an, "fp64", False, log, stall)
+    restart rule as the mixed solve (``stall`` = 0: one CG run).
+    ``tree_order``: solve on the cloud in its tree's point order (the
+    solution comes back in the input order)."""
+    return _cgnr(wt, width, tol, b, max_iter, max_restarts, an, "fp64", False, log, stall,
+                 tree_order)
 
 
 def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6, b: float = 0.5,
                      max_iter: int = 5000, max_restarts: int = 10, an: float = 1.0,
-                     plan: bool = True, log=None, stall: int = 1500):
+                     plan: bool = True, log=None, stall: int = 1500, tree_order: bool = True):
     """CGNR whose operator applications run in FP32c while every vector, dot
     product and update stays FP64; when the recurrence converges -- or
     stagnates (``stall`` iterations without a 10 % gain: the FP32c forward and
@@ -247,5 +262,7 @@ def cgnr_solve_mixed(wt: WindingNumberTreecode, width: float, tol: float = 1e-6,
     the FP64 operator and exact adjoint and CG restarts on the correction from
     that true residual (iterative refinement; the correction is small, so its
     floor is lower) until the FP64 test passes.  ``plan``: the FP32c adjoint
-    runs through an adjoint plan (its build is inside the timed region)."""
-    return _cgnr(wt, width, tol, b, max_iter, max_restarts, an, "fp32c", plan, log, stall)
+    runs through an adjoint plan (its build is inside the timed region).
+    ``tree_order``: as ``cgnr_solve``."""
+    return _cgnr(wt, width, tol, b, max_iter, max_restarts, an, "fp32c", plan, log, stall,
+                 tree_order)

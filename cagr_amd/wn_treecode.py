@@ -215,7 +215,25 @@ class WindingNumberTreecode:
             check(self.L.cagr_split_walks(self._h, float(piece_cost), int(max_pieces),
                                           cnt.ctypes.data_as(ct.c_void_p), _stream(self.device)))
         self.walk_pieces = (int(cnt[0]), int(cnt[1]))   # (pieces, waves cut)
+        self._balance_args = (op, float(piece_cost), int(max_pieces))
         return cost
+
+    def in_tree_order(self):
+        """This cloud as a new treecode whose input order is this tree's point
+        order, and that order: row s of the clone is row perm[s] of this cloud.
+        The clone's tree is this one (the same Morton codes, the same order
+        inside every leaf), so its permutation is the identity and its
+        operators read and write [N, .] vectors without gathers or scatters
+        through it; a solver that keeps its vectors in this order permutes
+        only its right-hand side and its solution.  Walks cut into pieces
+        here are cut again on the clone (costed with unit attributes)."""
+        perm = torch.from_numpy(self.permutation().astype(np.int64)).to(self.device)
+        t = WindingNumberTreecode(self.points[perm], self.tree_depth, theta=self.theta,
+                                  leaf_size=self.leaf_size, precision=self.precision)
+        if self.walk_pieces[0] > 0:
+            op, piece_cost, max_pieces = self._balance_args
+            t.balance(op, piece_cost=piece_cost, max_pieces=max_pieces)
+        return t, perm
 
     def unbalance(self) -> None:
         """Back to Morton dispatch order, uncut walks."""

@@ -12,6 +12,8 @@
   operator.
 * CG step: the library's fused vector work of one CG iteration against the
   same recurrence in torch FP64.
+* tree order: the clone of a cloud in its tree's point order has the
+  identity permutation and the same operator (the forward bit for bit).
 """
 import numpy as np
 import pytest
@@ -196,6 +198,34 @@ def test_cgnr_fp64_and_mixed_reach_tolerance():
     fit64 = float(torch.linalg.norm(b - op.A(mu64, "fp64")))
     fit32 = float(torch.linalg.norm(b - op.A(mu32, "fp64")))
     assert fit32 <= 1.05 * fit64
+
+
+@pytest.mark.parametrize("pieces", [False, True])
+def test_tree_order_clone_is_the_same_operator(pieces):
+    # the clone's tree is the original's: the same waves (and, balanced, the
+    # same walk pieces), so every forward result is the original's, permuted,
+    # bit for bit; the adjoints add their far terms with FP64 atomics, in any
+    # order, so they agree to rounding
+    say("tree order", pieces)
+    pts = cloud(20000, 6)
+    n = pts.shape[0]
+    wt = WindingNumberTreecode(pts, leaf_size=32, precision="fp32c")
+    if pieces:
+        wt.balance("A2")
+    t2, perm = wt.in_tree_order()
+    assert torch.equal(torch.from_numpy(t2.permutation().astype(np.int64)), torch.arange(n))
+    assert t2.walk_pieces == wt.walk_pieces
+    rng = np.random.default_rng(7)
+    mu = torch.from_numpy(rng.normal(size=(n, 3))).to(DEV)
+    w = torch.full((n,), 0.01, dtype=torch.float64, device=DEV)
+    ones = torch.ones(n, dtype=torch.float64, device=DEV)
+    for prec in ("fp32c", "fp64"):
+        a = wt.apply("A2", mu, w, weights=ones, precision=prec)
+        b = t2.apply("A2", mu[perm], w, weights=ones, precision=prec)
+        assert torch.equal(a[perm], b), prec
+        ga = wt.apply_adjoint("A2", a, w, ones, precision=prec)
+        gb = t2.apply_adjoint("A2", a[perm], w, ones, precision=prec)
+        assert rel(gb, ga[perm]) <= 1e-12, prec
 
 
 @pytest.mark.parametrize("length", [3, 1000, 3_000_000])
